@@ -1,0 +1,300 @@
+"""CPU oracle of the oa-fdtd time step: a NumPy restatement of the reference engine.
+
+TEST INFRASTRUCTURE ONLY. Nothing in the product package imports this module; only ``tests/``,
+``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline leg use it, as the checker.
+
+It restates ``packages/oa-fdtd/src/oa_fdtd/reference.py:41-331`` (``NumpyEngine``): the same
+equations as vectorised slices and the same step order (``reference.py:251-266``):
+
+    H bulk -> CPML(H) -> PMC ghosts -> record H at (n+1/2)dt -> save E^n on dispersive edges
+    -> E bulk -> CPML(E) -> SIBC overwrite -> inject s((n+1/2)dt) -> ADE post -> record E at
+    (n+1)dt
+
+Pinning (what fixes each function independently of this restatement):
+
+- ``_h_update``/``_e_update``/``_ghosts``: PEC-cavity resonances (closed form), Yee dispersion
+  (closed form), leapfrog energy conservation (invariant), parallel-plate TEM line S21 (closed
+  form), and the reference's own field/probe vectors under ``tests/golden/``.
+- ``_pml``: CPML reflection below -60 dB against a longer domain, late-time energy decay.
+- ``_ade_post``: Debye line S21 against the analytic ABCD line; golden vectors.
+- ``_sibc``: copper line loss against the analytic slab-impedance line; golden vectors.
+- ``_inject``/``_record``/DFT: mismatched-port |S11| = 1/3, TEM line, golden vectors.
+- ``energy``: closed-box energy equals the analytic sum for a planted field (tests).
+"""
+
+from __future__ import annotations
+
+from typing import Any
+
+import numpy as np
+
+from open_analysis_amd.fdtd.grid import EPS0, MU0
+from open_analysis_amd.fdtd.model import Compiled
+from open_analysis_amd.fdtd.pml import E_TERMS, H_TERMS
+
+Rng = tuple[tuple[int, int], ...]
+
+
+def _sl(rng: Rng, axis: int = -1, shift: int = 0) -> tuple[Any, ...]:
+    """Index of the ``(B, z, y, x)`` block over ``rng`` (x, y, z), shifted along ``axis``."""
+    out: list[Any] = [slice(None)]
+    for a in (2, 1, 0):
+        lo, hi = rng[a]
+        s = shift if a == axis else 0
+        out.append(slice(lo + s, hi + s))
+    return tuple(out)
+
+
+def _along(v: np.ndarray, axis: int, rng: Rng) -> np.ndarray:
+    """1-D coefficient ``v`` restricted to ``rng`` and shaped to broadcast along ``axis``."""
+    lo, hi = rng[axis]
+    shape = [1, 1, 1, 1]
+    shape[3 - axis] = hi - lo
+    return v[lo:hi].reshape(shape)
+
+
+class OracleEngine:
+    """Leapfrog Yee stepping of a :class:`Compiled` problem on the CPU (default float64)."""
+
+    name = "numpy"
+
+    def __init__(self, c: Compiled, dtype: str = "float64", record_series: bool = False,
+                 max_steps: int | None = None) -> None:
+        self.c = c
+        self.dtype = np.dtype(dtype)
+        self.record_series = record_series
+        self.max_steps = max_steps
+        g, dt, b = c.grid, self.dtype, c.batch
+        shape = (b, *g.shape)
+        self.E = [np.zeros(shape, dt) for _ in range(3)]
+        self.H = [np.zeros(shape, dt) for _ in range(3)]
+        self.ca = [c.ca_tab[i][None].astype(dt) for i in c.idx]
+        self.cb = [c.cb_tab[i][None].astype(dt) for i in c.idx]
+        self.dtmu = dt.type(c.dt / MU0)
+        self.ip = [g.inv_primal(a).astype(dt) for a in range(3)]
+        self.idd = [g.inv_dual(a).astype(dt) for a in range(3)]
+        self.full: Rng = tuple((1, g.n[a] + 2) for a in range(3))
+        self.psi = [np.zeros((b, *k.shape), dt) for k in c.pml.corrections]
+        self.pmc = [(a, s) for a in range(3) for s in (0, 1) if c.walls[2 * a + s] == "pmc"]
+        self.step_count = 0
+        self.t_e = [p for p in c.probes if p["kind"] == "E"]
+        self.t_h = [p for p in c.probes if p["kind"] == "H"]
+        nf = c.frequencies.size
+        self.acc = {"E": np.zeros((nf, b, len(self.t_e)), np.complex128),
+                    "H": np.zeros((nf, b, len(self.t_h)), np.complex128)}
+        self._series: dict[str, list[np.ndarray]] = {"E": [], "H": []}
+        self.omega = 2.0 * np.pi * c.frequencies
+        self._setup_lists()
+
+    # ---- compact-list physics (reference.py:82-149)
+    def _flat(self, f: np.ndarray) -> np.ndarray:
+        return f.reshape(self.c.batch, -1)
+
+    def _setup_lists(self) -> None:
+        c, dt, b = self.c, self.dtype, self.c.batch
+        self.ade = []
+        if c.ade is not None:
+            for ci, flat, beta, cbe in zip(c.ade.comp, c.ade.flat, c.ade.beta, c.ade.cbe,
+                                           strict=True):
+                self.ade.append({"c": ci, "flat": flat, "beta": beta.astype(dt),
+                                 "cbe": cbe.astype(dt),
+                                 "J": np.zeros((c.ade.tau.size, b, flat.size), dt),
+                                 "eold": np.zeros((b, flat.size), dt)})
+            self.kp = c.ade.kp.astype(dt)
+        sb = c.sibc
+        if sb is not None:
+            ne = sb.flat.size
+            self.sib_j = np.zeros((b, ne), dt)
+            self.sib_v = np.zeros((sb.cp.size, b, ne), dt)
+            self.sib_alpha = (sb.rp[sb.mat] * sb.cp[None, :]).astype(dt)  # (ne, P)
+
+    def _ade_pre(self) -> None:
+        for st in self.ade:
+            st["eold"][...] = self._flat(self.E[st["c"]])[:, st["flat"]]
+
+    def _ade_post(self) -> None:
+        for st in self.ade:
+            ef = self._flat(self.E[st["c"]])
+            jt = 0.5 * np.einsum("p,pbe->be", 1.0 + self.kp, st["J"])
+            en = ef[:, st["flat"]] - st["cbe"][None, :] * jt
+            st["J"][...] = (self.kp[:, None, None] * st["J"]
+                            + st["beta"][:, None, :] * (en - st["eold"])[None])
+            ef[:, st["flat"]] = en
+
+    def _sibc(self) -> None:
+        sb = self.c.sibc
+        if sb is None:
+            return
+        b, dt = self.c.batch, self.dtype
+        j = np.zeros((b, sb.flat.size), dt)
+        for s in range(2):
+            for hc in range(3):
+                ids = np.nonzero((sb.hcomp[:, s] == hc) & (sb.hsign[:, s] != 0))[0]
+                if ids.size:
+                    hf = self._flat(self.H[hc])
+                    j[:, ids] += sb.hsign[ids, s].astype(dt)[None, :] * hf[:, sb.hflat[ids, s]]
+        dj = j - self.sib_j
+        self.sib_v[...] = (self.sib_alpha.T[:, None, :] * dj[None]
+                           + sb.gp.astype(dt)[:, None, None] * self.sib_v)
+        out = sb.r0[sb.mat].astype(dt)[None, :] * j + self.sib_v.sum(axis=0)
+        self.sib_j = j
+        for ci in range(3):
+            ids = np.nonzero(sb.comp == ci)[0]
+            if ids.size:
+                self._flat(self.E[ci])[:, sb.flat[ids]] = out[:, ids]
+
+    # ---- bulk updates (reference.py:152-180)
+    def _h_update(self) -> None:
+        g = self.c.grid
+        for ci in range(3):
+            rng = g.valid("H", ci)
+            total = None
+            for c_, a, b_, sign in H_TERMS:
+                if c_ != ci:
+                    continue
+                d = (self.E[b_][_sl(rng, a, 1)] - self.E[b_][_sl(rng)]) * _along(self.ip[a], a, rng)
+                total = (d if sign > 0 else -d) if total is None else (
+                    total + d if sign > 0 else total - d)
+            self.H[ci][_sl(rng)] += self.dtmu * total
+
+    def _e_update(self) -> None:
+        rng = self.full
+        for ci in range(3):
+            total = None
+            for c_, a, b_, sign in E_TERMS:
+                if c_ != ci:
+                    continue
+                d = (self.H[b_][_sl(rng)] - self.H[b_][_sl(rng, a, -1)]) * _along(self.idd[a], a, rng)
+                total = (d if sign > 0 else -d) if total is None else (
+                    total + d if sign > 0 else total - d)
+            blk = _sl(rng)
+            self.E[ci][blk] = self.ca[ci][blk] * self.E[ci][blk] + self.cb[ci][blk] * total
+
+    def _pml(self, kind: str) -> None:
+        c, dt = self.c, self.dtype
+        for corr, psi in zip(c.pml.corrections, self.psi, strict=True):
+            if corr.kind != kind:
+                continue
+            co = c.pml.coeffs[corr.a]
+            rng, a = corr.box, corr.a
+            if kind == "E":
+                src = self.H[corr.b]
+                d = (src[_sl(rng)] - src[_sl(rng, a, -1)]) * _along(self.idd[a], a, rng)
+                bb, cc, kk = co.eb, co.ec, co.ek
+            else:
+                src = self.E[corr.b]
+                d = (src[_sl(rng, a, 1)] - src[_sl(rng)]) * _along(self.ip[a], a, rng)
+                bb, cc, kk = co.hb, co.hc, co.hk
+            psi *= _along(bb.astype(dt), a, rng)
+            psi += _along(cc.astype(dt), a, rng) * d
+            upd = _along(kk.astype(dt), a, rng) * d + psi
+            if kind == "E":
+                self.E[corr.c][_sl(rng)] += corr.sign * self.cb[corr.c][_sl(rng)] * upd
+            else:
+                self.H[corr.c][_sl(rng)] += corr.sign * self.dtmu * upd
+
+    def _ghosts(self) -> None:
+        g = self.c.grid
+        for a, s in self.pmc:
+            dst, src = (0, 1) if s == 0 else (g.n[a] + 1, g.n[a])
+            for comp in range(3):
+                if comp == a:
+                    continue
+                h = self.H[comp]
+                d_ix: list[Any] = [slice(None)] * 4
+                s_ix: list[Any] = [slice(None)] * 4
+                d_ix[3 - a], s_ix[3 - a] = dst, src
+                h[tuple(d_ix)] = -h[tuple(s_ix)]
+
+    def _inject(self, step: int) -> None:
+        c = self.c
+        for ci, ch in enumerate(c.channels):
+            w = float(ch["waveform"](np.array([(step + 0.5) * c.dt]))[0])
+            f = self._flat(self.E[int(ch["comp"])])
+            add = (c.drive[:, ci : ci + 1] * w) * ch["amps"][None, :]
+            f[:, ch["flat"]] += add.astype(self.dtype)
+
+    def _record(self, kind: str, t: float) -> None:
+        probes = self.t_e if kind == "E" else self.t_h
+        if not probes:
+            return
+        arrs = self.E if kind == "E" else self.H
+        b = self.c.batch
+        vals = np.empty((b, len(probes)))
+        for pi, p in enumerate(probes):
+            f = self._flat(arrs[p["comp"]])
+            vals[:, pi] = (f[:, p["flat"]].astype(np.float64) * p["weights"][None, :]).sum(axis=1)
+        if self.record_series:
+            self._series[kind].append(vals)
+        ph = np.exp(-1j * self.omega * t) * self.c.dt
+        self.acc[kind] += ph[:, None, None] * vals[None, :, :]
+
+    def step(self, n: int = 1) -> None:
+        if self.max_steps is not None and self.step_count + n > self.max_steps:
+            raise ValueError("step count exceeds max_steps (wave table length)")
+        dt = self.c.dt
+        for _ in range(n):
+            k = self.step_count
+            self._h_update()
+            self._pml("H")
+            self._ghosts()
+            self._record("H", (k + 0.5) * dt)
+            self._ade_pre()
+            self._e_update()
+            self._pml("E")
+            self._sibc()
+            self._inject(k)
+            self._ade_post()
+            self._record("E", (k + 1) * dt)
+            self.step_count += 1
+
+    # ---- observables (reference.py:269-331)
+    def _weights(self, kind: str, ci: int) -> np.ndarray:
+        g = self.c.grid
+        out = np.ones((1, 1, 1, 1))
+        for a in range(3):
+            node = (a != ci) if kind == "E" else (a == ci)
+            w = g.node_weights(a) if node else g.cell_weights(a)
+            shape = [1, 1, 1, 1]
+            shape[3 - a] = w.size
+            out = out * w.reshape(shape)
+        return out
+
+    def energy(self) -> float:
+        c = self.c
+        total = 0.0
+        for ci in range(3):
+            eps = (c.eps_tab[c.idx[ci]] * EPS0)[None]
+            total += 0.5 * float(np.sum(eps * self._weights("E", ci)
+                                        * self.E[ci].astype(np.float64) ** 2))
+            total += 0.5 * MU0 * float(np.sum(self._weights("H", ci)
+                                              * self.H[ci].astype(np.float64) ** 2))
+        return total
+
+    def get_fields(self) -> dict[str, np.ndarray]:
+        names = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")
+        return {n: a.astype(np.float64) for n, a in zip(names, (*self.E, *self.H), strict=True)}
+
+    def results(self) -> dict[str, np.ndarray]:
+        out: dict[str, np.ndarray] = {}
+        for kind, probes in (("E", self.t_e), ("H", self.t_h)):
+            for pi, p in enumerate(probes):
+                out[p["name"]] = self.acc[kind][:, :, pi].T.copy()
+        return out
+
+    def series(self) -> dict[str, np.ndarray]:
+        out: dict[str, np.ndarray] = {}
+        for kind, probes in (("E", self.t_e), ("H", self.t_h)):
+            if probes and self._series[kind]:
+                arr = np.stack(self._series[kind], axis=-1)  # (B, P, steps)
+                for pi, p in enumerate(probes):
+                    out[p["name"]] = arr[:, pi, :]
+        return out
+
+    def peak_bytes(self) -> int:
+        return 0
+
+
+def oracle_factory(c: Compiled, dtype: str, record_series: bool, max_steps: int):
+    """Engine factory with the signature ``run(engine=...)`` accepts."""
+    return OracleEngine(c, dtype, record_series, max_steps), {"backend": "cpu"}

@@ -1,0 +1,124 @@
+"""Small problems shared by the golden-vector generator and the tests.
+
+Every builder takes a ``classes`` mapping (``Grid``, ``Model``, ``Waveform``, ``Region``,
+``Conductor``, ``LumpedPort``, ``SoftSource``, ``Probe``, ``CpmlSpec``) so that the identical
+problem can be built from the reference's classes (to write the vectors) and from this package's
+classes (to check against them).
+"""
+
+from __future__ import annotations
+
+from typing import Any
+
+import numpy as np
+
+Classes = dict[str, Any]
+
+
+def graded_lines(length: float, n: int, ratio: float) -> np.ndarray:
+    """``n`` cells over ``length`` with geometric growth ``ratio`` per cell (1 = uniform)."""
+    if ratio == 1.0:
+        return np.linspace(0.0, length, n + 1)
+    w = ratio ** np.arange(n)
+    return np.concatenate([[0.0], np.cumsum(w) / w.sum() * length])
+
+
+def gpu_cpu_problem(k: Classes, batch: int = 2) -> Any:
+    """The reference's GPU-vs-CPU problem (tests/test_gpu_cpu.py:27-48): graded 24x20x22 mesh,
+    lossy eps 4.4 slab, PEC patch, soft source, lumped port, E and H probes, 4-cell CPML on x,
+    PMC on y, PEC on z, batch drive [[1, 1], [0, -2]]."""
+    lines = (graded_lines(0.024, 24, 1.03), graded_lines(0.02, 20, 1.0),
+             graded_lines(0.022, 22, 0.97))
+    m = k["Model"](k["Grid"](*lines), k["Waveform"]("gauss_mod", 10e9, 20e9),
+                   np.array([2e9, 6e9, 10e9, 14e9]),
+                   walls=("cpml", "cpml", "pmc", "pmc", "pec", "pec"))
+    m.cpml = k["CpmlSpec"](cells=4)
+    m.regions.append(k["Region"]((0, 0, 0.004), (0.024, 0.02, 0.012), eps_r=4.4, sigma=0.02))
+    m.regions.append(k["Region"]((0.008, 0.006, 0.012), (0.016, 0.014, 0.0121), pec=True))
+    m.sources.append(k["SoftSource"]("s", 2, (0.012, 0.01, 0.006), (0.012, 0.01, 0.006)))
+    m.ports.append(k["LumpedPort"]("p", 2, (0.006, 0.01, 0.0), (0.006, 0.01, 0.004), 50.0))
+    m.probes.append(k["Probe"]("e", "E", 2, (0.016, 0.01, 0.002), (0.016, 0.01, 0.002)))
+    m.probes.append(k["Probe"]("h", "H", 1, (0.016, 0.01, 0.002), (0.016, 0.01, 0.002)))
+    if batch > 1:
+        m.drive = np.array([[1.0, 1.0], [0.0, -2.0]])[:batch]
+    return m
+
+
+def debye_sibc_line(k: Classes) -> Any:
+    """Parallel-plate line: single-pole Debye filling (eps_inf 4.0, delta 0.4, tau 8 ps), SIBC
+    copper plates (5.8e7 S/m, 0.25 mm slabs), lumped ports at both ends, 3-cell CPML along y,
+    PMC on x, PEC on z."""
+    w, h, t = 0.002, 0.001, 0.25e-3
+    nx, nzg, ny, dy = 4, 4, 24, 0.25e-3
+    dz = h / nzg
+    zl = [0.0, t] + [t + (i + 1) * dz for i in range(nzg)] + [2 * t + h]
+    g = k["Grid"](np.linspace(0, w, nx + 1), np.arange(ny + 1) * dy, np.array(zl))
+    length = ny * dy
+    m = k["Model"](g, k["Waveform"]("gauss_mod", 10e9, 20e9), np.linspace(2e9, 20e9, 7),
+                   walls=("pmc", "pmc", "cpml", "cpml", "pec", "pec"))
+    m.cpml = k["CpmlSpec"](cells=3)
+    m.regions.append(k["Region"]((0, 0, t), (w, length, t + h), 4.0, poles=((0.4, 8e-12),)))
+    m.conductors.append(k["Conductor"]((-1, -1, -1), (w + 1, length + 1, t), 5.8e7, t))
+    m.conductors.append(k["Conductor"]((-1, -1, t + h), (w + 1, length + 1, 3 * t + h), 5.8e7, t))
+    y1, y2 = 6 * dy, 18 * dy
+    m.ports.append(k["LumpedPort"]("p1", 2, (0, y1, t), (w, y1, t + h), 40.0))
+    m.ports.append(k["LumpedPort"]("p2", 2, (0, y2, t), (w, y2, t + h), 40.0))
+    return m
+
+
+def cpml_box(k: Classes) -> Any:
+    """Vacuum 12x10x14 box, 3-cell CPML on every face, point soft sources on all three axes,
+    'line' and 'mean' probes of E and H."""
+    d = 0.5e-3
+    g = k["Grid"](np.arange(13) * d, np.arange(11) * d, np.arange(15) * d)
+    m = k["Model"](g, k["Waveform"]("gauss", 0.0, 30e9), np.array([3e9, 9e9, 15e9]),
+                   walls=("cpml",) * 6)
+    m.cpml = k["CpmlSpec"](cells=3)
+    p = (5 * d, 4 * d, 6 * d)
+    for ax in range(3):
+        m.sources.append(k["SoftSource"](f"s{ax}", ax, p, p, 1.0 + ax))
+    q_lo, q_hi = (6 * d, 5 * d, 7 * d), (7 * d, 6 * d, 8 * d)
+    m.probes.append(k["Probe"]("ex", "E", 0, q_lo, q_hi, "line"))
+    m.probes.append(k["Probe"]("hz", "H", 2, q_lo, q_hi, "mean"))
+    m.probes.append(k["Probe"]("hy", "H", 1, q_lo, q_hi, "line"))
+    return m
+
+
+def microstrip(k: Classes, nx: int = 160, ny: int = 80, nz: int = 48, d: float = 0.15e-3,
+               npml: int = 8, sub: int = 2, strip_w: int = 4, strip_len: int = 130,
+               eps_r: float = 4.3, r0: float = 1e-6, freqs: Any = None) -> Any:
+    """Plain microstrip (the 160x80x48, 0.15 mm, 8000-step workload at its defaults): PEC
+    ground sheet ``npml + 2`` cells above the bottom, ``sub``-cell eps_r substrate, PEC strip
+    ``strip_w`` x ``strip_len`` cells along x, 50-ohm lumped ports (z) at both strip ends,
+    CPML (m = 3, R0 = 1e-6) on all faces."""
+    g = k["Grid"](np.arange(nx + 1) * d, np.arange(ny + 1) * d, np.arange(nz + 1) * d)
+    if freqs is None:
+        freqs = np.linspace(1e9, 10e9, 401)
+    m = k["Model"](g, k["Waveform"]("gauss_mod", 6e9, 12e9), np.asarray(freqs),
+                   walls=("cpml",) * 6, courant=0.99)
+    m.cpml = k["CpmlSpec"](cells=npml, m=3.0, r0=r0)
+    zg = (npml + 2) * d
+    zs = zg + sub * d
+    m.regions.append(k["Region"]((0, 0, zg), (nx * d, ny * d, zs), eps_r))
+    m.regions.append(k["Region"]((0, 0, zg), (nx * d, ny * d, zg), pec=True))
+    x0 = ((nx - strip_len) // 2) * d
+    x1 = x0 + strip_len * d
+    y0 = ((ny - strip_w) // 2) * d
+    y1 = y0 + strip_w * d
+    m.regions.append(k["Region"]((x0, y0, zs), (x1, y1, zs), pec=True))
+    m.ports.append(k["LumpedPort"]("p1", 2, (x0, y0, zg), (x0, y1, zs), 50.0))
+    m.ports.append(k["LumpedPort"]("p2", 2, (x1, y0, zg), (x1, y1, zs), 50.0))
+    return m
+
+
+def microstrip_small(k: Classes) -> Any:
+    return microstrip(k, nx=40, ny=20, nz=16, npml=4, strip_w=4, strip_len=24,
+                      freqs=np.linspace(1e9, 10e9, 9))
+
+
+CASES = {
+    "gpu_cpu": (gpu_cpu_problem, 120),
+    "debye_sibc_line": (debye_sibc_line, 150),
+    "cpml_box": (cpml_box, 100),
+    "microstrip_small": (microstrip_small, 120),
+}

@@ -1,0 +1,87 @@
+"""Write golden vectors from the reference oa-fdtd NumPy engine (run in the development
+container only; the GPU machine never needs the reference).
+
+    python tests/golden/make_golden.py /root/reference
+
+For each small case it stores, in ``tests/golden/<case>.npz`` (no pickles):
+
+- the reference ``compile_model`` outputs (material table, indices, ca/cb, injection amplitudes,
+  probe weights, ADE and SIBC lists, dt);
+- the reference ``NumpyEngine`` (float64) fields after ``steps`` steps and its probe DFTs.
+
+The problems are built from plain numbers below, so the same problems can be rebuilt with this
+package's classes in the tests (``tests/cases.py``).
+"""
+
+from __future__ import annotations
+
+import sys
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+sys.path.insert(0, str(HERE.parent))
+
+
+def main(ref_root: str) -> None:
+    sys.path.insert(0, str(Path(ref_root) / "packages" / "oa-fdtd" / "src"))
+    import oa_fdtd.model as rm  # noqa: PLC0415
+    from oa_fdtd.grid import Grid as RGrid  # noqa: PLC0415
+    from oa_fdtd.pml import CpmlSpec as RSpec  # noqa: PLC0415
+    from oa_fdtd.reference import NumpyEngine  # noqa: PLC0415
+
+    import cases  # noqa: PLC0415
+
+    classes = {
+        "Grid": RGrid, "Model": rm.Model, "Waveform": rm.Waveform, "Region": rm.Region,
+        "Conductor": rm.Conductor, "LumpedPort": rm.LumpedPort, "SoftSource": rm.SoftSource,
+        "Probe": rm.Probe, "CpmlSpec": RSpec,
+    }
+    for name, (builder, steps) in cases.CASES.items():
+        model = builder(classes)
+        c = rm.compile_model(model, "float64")
+        eng = NumpyEngine(c, "float64")
+        eng.step(steps)
+        out: dict[str, np.ndarray] = {
+            "steps": np.array(steps), "dt": np.array(c.dt),
+            "eps_tab": c.eps_tab, "sig_tab": c.sig_tab, "ca_tab": c.ca_tab, "cb_tab": c.cb_tab,
+            "drive": c.drive,
+        }
+        for ci in range(3):
+            out[f"idx{ci}"] = c.idx[ci]
+        for ci, ch in enumerate(c.channels):
+            out[f"ch{ci}_flat"] = np.asarray(ch["flat"], np.int64)
+            out[f"ch{ci}_amps"] = np.asarray(ch["amps"], np.float64)
+        for pi, p in enumerate(c.probes):
+            out[f"pr{pi}_flat"] = np.asarray(p["flat"], np.int64)
+            out[f"pr{pi}_w"] = np.asarray(p["weights"], np.float64)
+        if c.ade is not None:
+            out["ade_kp"] = c.ade.kp
+            for n, ci in enumerate(c.ade.comp):
+                out[f"ade{ci}_flat"] = c.ade.flat[n]
+                out[f"ade{ci}_beta"] = c.ade.beta[n]
+                out[f"ade{ci}_cbe"] = c.ade.cbe[n]
+        if c.sibc is not None:
+            sb = c.sibc
+            for k in ("comp", "flat", "hcomp", "hflat", "hsign", "mat", "r0", "rp", "cp", "gp"):
+                out[f"sibc_{k}"] = np.asarray(getattr(sb, k))
+        # fields: checksums of every component plus values at a fixed sample of cells
+        rng = np.random.default_rng(1234)
+        for n, a in zip(("Ex", "Ey", "Ez", "Hx", "Hy", "Hz"), (*eng.E, *eng.H), strict=True):
+            f = a.astype(np.float64).ravel()
+            pick = np.sort(rng.choice(f.size, size=min(4000, f.size), replace=False))
+            out[f"field_{n}_pick"] = pick.astype(np.int64)
+            out[f"field_{n}_val"] = f[pick]
+            out[f"field_{n}_sum"] = np.array(f.sum())
+            out[f"field_{n}_sq"] = np.array((f * f).sum())
+            out[f"field_{n}_abs"] = np.array(np.abs(f).sum())
+        for k, v in eng.results().items():
+            out["dft_" + k.replace(":", "_")] = v
+        out["energy"] = np.array(eng.energy())
+        np.savez_compressed(HERE / f"{name}.npz", **out)
+        print(name, "steps", steps, "dt", c.dt, "table", c.eps_tab.size)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")

@@ -1,0 +1,88 @@
+"""The host compile step and the CPU oracle against the reference's own vectors.
+
+The vectors in ``tests/golden/*.npz`` come from the reference ``compile_model`` and
+``NumpyEngine`` (float64), imported in the development container by ``make_golden.py``.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+from golden_io import build, field_errors, load
+
+from open_analysis_amd.fdtd.model import compile_model
+from oracle import OracleEngine
+
+CASES = ["gpu_cpu", "debye_sibc_line", "cpml_box", "microstrip_small"]
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_compile_matches_reference(name: str) -> None:
+    gold = load(name)
+    model, _ = build(name)
+    c = compile_model(model)
+    assert c.dt == pytest.approx(float(gold["dt"]), rel=1e-15)
+    for key in ("eps_tab", "sig_tab", "ca_tab", "cb_tab"):
+        np.testing.assert_allclose(getattr(c, key), gold[key], rtol=1e-13, atol=0, err_msg=key)
+    for ci in range(3):
+        assert np.array_equal(c.idx[ci], gold[f"idx{ci}"]), f"idx{ci}"
+    np.testing.assert_array_equal(c.drive, gold["drive"])
+    for ci, ch in enumerate(c.channels):
+        np.testing.assert_array_equal(ch["flat"], gold[f"ch{ci}_flat"])
+        np.testing.assert_allclose(ch["amps"], gold[f"ch{ci}_amps"], rtol=1e-13)
+    for pi, p in enumerate(c.probes):
+        np.testing.assert_array_equal(p["flat"], gold[f"pr{pi}_flat"])
+        np.testing.assert_allclose(p["weights"], gold[f"pr{pi}_w"], rtol=1e-13)
+    if "ade_kp" in gold:
+        assert c.ade is not None
+        np.testing.assert_allclose(c.ade.kp, gold["ade_kp"], rtol=1e-15)
+        for n, ci in enumerate(c.ade.comp):
+            np.testing.assert_array_equal(c.ade.flat[n], gold[f"ade{ci}_flat"])
+            np.testing.assert_allclose(c.ade.beta[n], gold[f"ade{ci}_beta"], rtol=1e-13)
+            np.testing.assert_allclose(c.ade.cbe[n], gold[f"ade{ci}_cbe"], rtol=1e-13)
+    else:
+        assert c.ade is None
+    if "sibc_flat" in gold:
+        sb = c.sibc
+        assert sb is not None
+        for k in ("comp", "flat", "hcomp", "hflat", "mat"):
+            np.testing.assert_array_equal(getattr(sb, k), gold[f"sibc_{k}"], err_msg=k)
+        np.testing.assert_allclose(sb.hsign, gold["sibc_hsign"], rtol=1e-15)
+        # NNLS ladder: same problem statement; SciPy versions may differ in the last bits
+        np.testing.assert_allclose(sb.r0, gold["sibc_r0"], rtol=1e-8)
+        np.testing.assert_allclose(sb.rp, gold["sibc_rp"], rtol=1e-6, atol=1e-9 * sb.rp.max())
+        np.testing.assert_allclose(sb.cp, gold["sibc_cp"], rtol=1e-15)
+        np.testing.assert_allclose(sb.gp, gold["sibc_gp"], rtol=1e-15)
+    else:
+        assert c.sibc is None
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_oracle_matches_reference_engine(name: str) -> None:
+    """Oracle float64 fields, probe DFTs and energy after N steps vs the reference engine."""
+    gold = load(name)
+    model, steps = build(name)
+    c = compile_model(model)
+    eng = OracleEngine(c, "float64")
+    eng.step(steps)
+    err = field_errors(eng.get_fields(), gold)
+    assert max(err.values()) < 1e-9, err
+    for k, v in eng.results().items():
+        ref = gold["dft_" + k.replace(":", "_")]
+        assert np.abs(v - ref).max() <= 1e-10 * np.abs(ref).max(), k
+    assert eng.energy() == pytest.approx(float(gold["energy"]), rel=1e-10)
+
+
+def test_planted_mistake_is_caught() -> None:
+    """A wrong CPML sign in the oracle must fail the golden comparison (the check has teeth)."""
+    gold = load("cpml_box")
+    model, steps = build("cpml_box")
+    c = compile_model(model)
+    eng = OracleEngine(c, "float64")
+    corr = list(c.pml.corrections)
+    k0 = corr[0]
+    corr[0] = type(k0)(k0.kind, k0.c, k0.a, k0.b, -k0.sign, k0.box, k0.side)
+    object.__setattr__(c.pml, "corrections", tuple(corr))
+    eng.step(steps)
+    err = field_errors(eng.get_fields(), gold)
+    assert max(err.values()) > 1e-6
