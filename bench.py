@@ -1,0 +1,167 @@
+"""Throughput of the HIP oa-fdtd engine on one MI355X, with its HBM roofline and the CPU
+oracle timed beside it.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--dtype float32]
+
+Prints one JSON line. ``value`` is Mcell-updates/s = nx*ny*nz*B*K / wall (the reference's cell
+rate definition, packages/oa-fdtd/bench/throughput.py:34-41 and bench/pcb_throughput.py:66-72),
+with every input resident in HBM before the timed region. ``roofline`` is for the fused step
+kernel ``yee_step``: its algorithmic bytes per launch (six field components read and written,
+the 3 x uint16 material index, the Debye current and the CPML psi of every cell that has them)
+divided by its mean launch time from HIP events on the engine's stream. ``cpu_baseline`` is the
+float64 NumPy oracle stepping the same compiled problem on one host core.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+S_FIELD = {"float32": 4, "float64": 8, "float16": 2, "bfloat16": 2}
+DT_NAME = {"float32": "f32", "float64": "f64", "float16": "f16-storage/f32", "bfloat16": "bf16-storage/f32"}
+
+
+def build_workload(name: str):
+    from open_analysis_amd.fdtd import workloads as W
+
+    if name == "stripline_stack":
+        m, eps = W.stripline_stack()
+        desc = ("six-layer coupled stripline pair, 800x400x200 cells at 0.05 mm, single-pole "
+                "Debye layers (eps_inf U(3.0,4.6) seed 7, delta 0.4, tau 8 ps), SIBC copper "
+                "(5.8e7 S/m) planes and strips, 10-cell CPML on all faces, 4 lumped ports, "
+                "801 DFT points 1-20 GHz")
+        return m, desc
+    if name == "dispersive_microstrip":
+        return W.dispersive_microstrip(), ("microstrip 400x200x100 at 0.1 mm, Debye substrate, "
+                                           "SIBC copper, 10-cell CPML, 801 points 1-20 GHz")
+    if name == "microstrip":
+        return W.microstrip(), "microstrip 160x80x48 at 0.15 mm, 8-cell CPML, 401 points 1-10 GHz"
+    if name == "vacuum256":
+        return W.vacuum_cube(), "256^3 vacuum, 10-cell CPML (the reference throughput bench)"
+    raise SystemExit(f"unknown workload {name!r}")
+
+
+def algorithmic_bytes(c, dtype: str) -> dict[str, float]:
+    """Bytes one step must move, per kernel (no re-reads, no padding, no list metadata)."""
+    nx, ny, nz = c.grid.n
+    n = nx * ny * nz
+    b = c.batch
+    s = S_FIELD[dtype]
+    r = 8 if dtype == "float64" else 4
+    n_ade = sum(f.size for f in c.ade.flat) if c.ade is not None else 0
+    n_pole = c.ade.tau.size if c.ade is not None else 0
+    n_psi = sum(int(np.prod(k.shape)) for k in c.pml.corrections)
+    yee = b * (n * 12 * s + n_ade * n_pole * 2 * r + n_psi * 2 * r) + n * 6
+    n_sibc = c.sibc.flat.size if c.sibc is not None else 0
+    n_p = c.sibc.cp.size if c.sibc is not None else 0
+    sibc = b * n_sibc * (n_p + 1) * 2 * r
+    return {"yee_step": float(yee), "sibc_step": float(sibc), "cells": float(n),
+            "ade_edges": float(n_ade), "psi": float(n_psi), "sibc_edges": float(n_sibc)}
+
+
+def cpu_baseline(c, seconds: float) -> dict:
+    """Float64 NumPy oracle on the same compiled problem: one warm-up step, then as many timed
+    steps as fit in about ``seconds`` (at least one)."""
+    from oracle import OracleEngine
+
+    eng = OracleEngine(c, "float64")
+    t0 = time.perf_counter()
+    eng.step(1)
+    one = time.perf_counter() - t0
+    k = max(1, min(50, int(seconds / max(one, 1e-6))))
+    t0 = time.perf_counter()
+    eng.step(k)
+    wall = time.perf_counter() - t0
+    nx, ny, nz = c.grid.n
+    rate = nx * ny * nz * c.batch * k / wall / 1e6
+    return {"value": rate, "unit": "Mcell-updates/s", "cores": 1, "kind": "port",
+            "sample": f"float64 NumPy oracle, same compiled grid, {k} timed steps after 1 warm-up "
+                      f"step ({wall:.1f} s), one core (numpy ufuncs are single-threaded; "
+                      f"nproc={os.cpu_count()}, OMP_NUM_THREADS={os.environ.get('OMP_NUM_THREADS')})"}
+
+
+def traffic_from_profile(workload: str, dtype: str):
+    """HBM bytes per yee_step launch measured with rocprofv3 PMC counters, if committed."""
+    p = ROOT / "profiles" / "traffic.json"
+    if not p.exists():
+        return None
+    data = json.loads(p.read_text())
+    return data.get(f"{workload}:{dtype}")
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--workload", default="stripline_stack")
+    ap.add_argument("--dtype", default="float32", choices=sorted(S_FIELD))
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {a.gpus}); the engine has no multi-GPU "
+              "path in this build.", file=sys.stderr)
+        return 2
+    from open_analysis_amd.fdtd.engine import HipEngine
+    from open_analysis_amd.fdtd.model import compile_model
+
+    model, desc = build_workload(a.workload)
+    t0 = time.perf_counter()
+    c = compile_model(model)
+    t_compile = time.perf_counter() - t0
+    print(f"[bench] compiled {a.workload} in {t_compile:.1f} s", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    eng = HipEngine(c, a.dtype, max_steps=a.warmup + a.steps + 8)
+    t_setup = time.perf_counter() - t0
+    eng.step(a.warmup)  # ends with a stream synchronisation
+    print(f"[bench] engine ready ({t_setup:.1f} s), warm-up done", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    tm = eng.step_timed(a.steps)  # ends with an event synchronisation
+    wall = time.perf_counter() - t0
+    nx, ny, nz = c.grid.n
+    n = nx * ny * nz
+    value = n * c.batch * a.steps / wall / 1e6
+    ab = algorithmic_bytes(c, a.dtype)
+    yee_avg_s = tm["yee_ms"] / max(tm["launches"], 1) / 1e3
+    achieved = ab["yee_step"] / yee_avg_s / 1e9
+    e = eng.energy()
+    out = {
+        "metric": "Mcell-updates/s", "value": value, "unit": "Mcell-updates/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": wall / a.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": DT_NAME[a.dtype], "data": "synthetic",
+        "config": {"workload": a.workload, "description": desc, "cells": n, "batch": c.batch,
+                   "grid": [nx, ny, nz], "ade_edges": int(ab["ade_edges"]),
+                   "psi_values": int(ab["psi"]), "sibc_edges": int(ab["sibc_edges"]),
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic_from_profile(a.workload, a.dtype),
+                     "kernel": "yee_step", "bytes_per_launch": ab["yee_step"],
+                     "avg_launch_ms": yee_avg_s * 1e3},
+        "rate_at_roofline": HBM_PEAK_GBS * 1e9 / (ab["yee_step"] / (n * c.batch)) / 1e6,
+        "step_breakdown_ms": {"yee_step": tm["yee_ms"] / a.steps,
+                              "other_kernels_and_gaps": (tm["wall_ms"] - tm["yee_ms"]) / a.steps},
+        "energy_J": e, "setup_s": {"compile_model": t_compile, "engine": t_setup},
+    }
+    del eng
+    if not a.no_cpu:
+        print("[bench] timing the CPU oracle", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(c, a.cpu_seconds)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

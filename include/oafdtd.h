@@ -1,0 +1,147 @@
+/*
+ * oafdtd.h -- C ABI of the MI355X (gfx950) oa-fdtd time-stepping engine.
+ *
+ * One opaque engine object owns every device buffer of one compiled problem. All functions
+ * return OAFD_OK (0) or a status code; oafd_last_error() gives the message. Host arrays passed
+ * in are copied; output arrays are caller-owned. No function takes or returns a framework type.
+ *
+ * Each entry point replaces one member of the reference CuPy engine
+ * (packages/oa-fdtd/src/oa_fdtd/engine.py), which is what a ctypes binding of that path calls:
+ *
+ *   oafd_create          CupyEngine.__init__           engine.py:73-119  (H2D copies :98-119)
+ *   oafd_step            CupyEngine.step               engine.py:418-437 (graph :390-401)
+ *   oafd_energy          CupyEngine.energy             engine.py:440-468
+ *   oafd_results         CupyEngine.results            engine.py:486-494
+ *   oafd_series_len/     CupyEngine.series             engine.py:496-503
+ *   oafd_series
+ *   oafd_get_field       CupyEngine.get_fields         engine.py:478-484
+ *   oafd_peak_bytes      CupyEngine.peak_bytes         engine.py:505-506
+ *   oafd_estimate_bytes  estimate_bytes                engine.py:29-40
+ *   oafd_device_count/   oa_accel probe / select       accel/probe.py:92, accel/policy.py:147
+ *   oafd_device_info
+ *   oafd_debug_inject_fault  CupyBackend.inject_fault  accel/backend.py:217-219
+ *   oafd_last_error      AccelError(ErrorInfo) text    accel/errors.py:17-34
+ *   oafd_destroy         engine object lifetime (CuPy pool frees on GC)
+ *   oafd_set_field       (tests: plant a field; the reference tests write NumpyEngine.E/H)
+ *   oafd_step_timed      (bench: per-kernel HIP-event timing, bench/throughput.py:34-41)
+ */
+#ifndef OAFDTD_H
+#define OAFDTD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define OAFD_OK 0
+#define OAFD_E_INVALID 1   /* bad descriptor or argument          -> ModelError / ValueError */
+#define OAFD_E_OOM 2       /* device allocation failed             -> ACCEL_OUT_OF_MEMORY   */
+#define OAFD_E_DEVICE 3    /* HIP runtime / kernel failure         -> ACCEL_DEVICE_ERROR    */
+#define OAFD_E_MAX_STEPS 4 /* step past the wave table (max_steps) -> ValueError            */
+#define OAFD_E_NO_DEVICE 5 /* no gfx950 device visible             -> ACCEL_NO_DEVICE       */
+
+/* field storage / arithmetic variants */
+#define OAFD_F32 0  /* FP32 storage, FP32 arithmetic (the drop-in)   */
+#define OAFD_F64 1  /* FP64 storage, FP64 arithmetic (GPU oracle)    */
+#define OAFD_F16 2  /* FP16 storage, FP32 arithmetic, H scaled by eta0 */
+#define OAFD_BF16 3 /* BF16 storage, FP32 arithmetic, H scaled by eta0 */
+
+#define OAFD_MAX_POLES 16
+#define OAFD_MAX_CORR 24
+
+/* Compiled problem (mirrors oa_fdtd.model.Compiled, model.py:245-269), in the reference
+ * storage layout (nz+2, ny+2, pitch) with x fastest; flat indices are per batch member. */
+typedef struct oafd_desc {
+    int32_t nx, ny, nz;      /* cells                                  */
+    int32_t X, Y, Z;         /* storage dims: pitch, ny+2, nz+2        */
+    int32_t batch;           /* excitations B                          */
+    int32_t n_mat;           /* rows of the material tables            */
+    double dt;               /* time step [s]                          */
+    double dtmu;             /* dt / mu0                               */
+    const uint16_t* idx[3];  /* per E component, (Z, Y, X) rows into the tables */
+    const double* ca;        /* (n_mat)                                */
+    const double* cb;        /* (n_mat)                                */
+    const double* eps;       /* (n_mat) relative permittivity (energy) */
+    /* Debye ADE folded into the E update: rows >= ade_first are dispersive */
+    int32_t n_poles;         /* 0 = no dispersive edges                */
+    int32_t ade_first;
+    int32_t ade_k0, ade_nk;  /* storage planes [ade_k0, ade_k0+ade_nk) holding dispersive edges */
+    const double* kp;        /* (n_poles) (1 - dt/2tau)/(1 + dt/2tau)  */
+    const double* beta;      /* (n_poles, n_mat - ade_first)           */
+    const double* inv_primal[3]; /* lengths X, Y, Z                    */
+    const double* inv_dual[3];
+    const double* node_w[3];
+    const double* cell_w[3];
+    int32_t walls[6];        /* xlo xhi ylo yhi zlo zhi: 0 pec, 1 pmc, 2 cpml */
+    /* CPML: per axis eb ec ek hb hc hk (storage length), NULL when the axis has none */
+    const double* pml[3][6];
+    int32_t n_corr;
+    const int32_t* corr;     /* (n_corr, 12): kind(0 E,1 H) c a b sign side i0 i1 j0 j1 k0 k1 */
+    /* SIBC face edges */
+    int32_t sibc_n, sibc_np, sibc_nmat;
+    const int32_t* sibc_comp;
+    const int64_t* sibc_flat;
+    const int32_t* sibc_hcomp;  /* (n, 2) */
+    const int64_t* sibc_hflat;  /* (n, 2) */
+    const double* sibc_hsign;   /* (n, 2) */
+    const int32_t* sibc_mat;
+    const double* sibc_r0;      /* (nmat)      */
+    const double* sibc_alpha;   /* (nmat, np)  R_p c_p */
+    const double* sibc_gp;      /* (np)        */
+    /* injection: unique targets (CSR over terms) */
+    int32_t inj_n, inj_terms, n_channels;
+    const int32_t* inj_comp;    /* (inj_n)     */
+    const int64_t* inj_flat;    /* (inj_n)     */
+    const int32_t* inj_start;   /* (inj_n + 1) */
+    const double* inj_amp;      /* (inj_terms) */
+    const int32_t* inj_chan;    /* (inj_terms) */
+    const double* drive;        /* (batch, n_channels) */
+    const double* wave;         /* (n_channels, max_steps) s((n + 1/2) dt) */
+    /* probes per kind (0 = E, 1 = H) */
+    int32_t n_probe[2];
+    const int32_t* probe_comp[2];
+    const int64_t* probe_off[2];  /* (n_probe + 1) */
+    const int64_t* probe_flat[2];
+    const double* probe_w[2];
+    int32_t n_freq;
+    const double* freqs;
+} oafd_desc;
+
+typedef struct oafd_engine oafd_engine;
+
+const char* oafd_version(void);
+int oafd_device_count(void);
+int oafd_device_info(int index, char* name, int name_len, char* arch, int arch_len,
+                     int64_t* total_bytes);
+
+int oafd_create(const oafd_desc* d, int dtype, int record_series, int64_t max_steps,
+                int use_graph, oafd_engine** out);
+int oafd_step(oafd_engine* e, int64_t n);
+int oafd_energy(oafd_engine* e, double* out_joules);
+/* out: (n_freq, batch, n_probe[kind], 2) float64 (re, im) */
+int oafd_results(oafd_engine* e, int kind, double* out);
+int64_t oafd_series_len(const oafd_engine* e);
+/* out: (steps, batch, n_probe[kind]) float64; cap = capacity in elements */
+int oafd_series(oafd_engine* e, int kind, double* out, int64_t cap);
+/* comp 0..2 = Ex Ey Ez, 3..5 = Hx Hy Hz; out/in: (batch, Z, Y, X) float64 */
+int oafd_get_field(oafd_engine* e, int comp, double* out);
+int oafd_set_field(oafd_engine* e, int comp, const double* in);
+int64_t oafd_step_count(const oafd_engine* e);
+int64_t oafd_peak_bytes(const oafd_engine* e);
+int64_t oafd_estimate_bytes(const oafd_desc* d, int dtype);
+/* Runs n eager steps with HIP events around every fused-update launch on the engine stream.
+ * out[0] = summed fused-kernel ms, out[1] = wall ms of the n steps (first to last event),
+ * out[2] = fused launches, out[3] = summed ms of the face/source/probe kernels. */
+int oafd_step_timed(oafd_engine* e, int64_t n, double* out);
+/* e == NULL: the next oafd_create returns `code` without touching the device (CPU tests);
+ * otherwise the next call on e returns `code`. */
+int oafd_debug_inject_fault(oafd_engine* e, int code);
+const char* oafd_last_error(const oafd_engine* e);
+void oafd_destroy(oafd_engine* e);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OAFDTD_H */

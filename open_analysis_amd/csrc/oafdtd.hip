@@ -1,0 +1,754 @@
+// Host side of the MI355X oa-fdtd engine: device buffers, launch sequence, HIP graph of one
+// 64-step chunk, FP64 running DFT, energy, transfers and the C ABI of include/oafdtd.h.
+//
+// Launch sequence of one step (the reference order, packages/oa-fdtd/src/oa_fdtd/engine.py:
+// 265-335, with the bulk/CPML/ADE/ghost work fused):
+//   yee_step (H, CPML-H, PMC ghosts, E, CPML-E, Debye ADE)  ->  sibc_step  ->  inject_step
+//   ->  probe_step (H at (n+1/2)dt and E at (n+1)dt into a 64-slot ring)
+// After each chunk: dft_accum per probe kind, advance_step.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/oafdtd.h"
+#include "kernels.hpp"
+
+namespace {
+
+constexpr int CHUNK = 64;
+constexpr double MU0 = 4e-7 * 3.14159265358979323846;
+constexpr double EPS0 = 1.0 / (MU0 * 299792458.0 * 299792458.0);
+constexpr double ETA0 = MU0 * 299792458.0;
+
+thread_local std::string g_err;
+int g_pending_create_fault = 0;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+struct HipFail {
+    int code;
+    std::string msg;
+};
+
+#define HIP_TRY(expr)                                                                            \
+    do {                                                                                         \
+        hipError_t _e = (expr);                                                                  \
+        if (_e != hipSuccess)                                                                    \
+            throw HipFail{_e == hipErrorOutOfMemory ? OAFD_E_OOM : OAFD_E_DEVICE,                \
+                          std::string(#expr) + ": " + hipGetErrorString(_e)};                     \
+    } while (0)
+
+template <typename T>
+std::vector<T> cast_vec(const double* src, size_t n) {
+    std::vector<T> out(n);
+    for (size_t i = 0; i < n; ++i) out[i] = (T)src[i];
+    return out;
+}
+
+}  // namespace
+
+struct oafd_engine {
+    virtual ~oafd_engine() = default;
+    virtual int step(int64_t n) = 0;
+    virtual int energy(double* out) = 0;
+    virtual int results(int kind, double* out) = 0;
+    virtual int series(int kind, double* out, int64_t cap) = 0;
+    virtual int get_field(int comp, double* out) = 0;
+    virtual int set_field(int comp, const double* in) = 0;
+    virtual int step_timed(int64_t n, double* out) = 0;
+    int64_t step_count = 0;
+    int64_t max_steps = 0;
+    int64_t peak = 0;
+    int64_t cur_bytes = 0;
+    int pending_fault = 0;
+    std::string err;
+};
+
+namespace {
+
+template <typename S, typename R, int RPWV>
+struct Engine final : oafd_engine {
+    // geometry
+    int nx, ny, nz, X, Y, Z, B;
+    long long bs;
+    double dt;
+    double hs;  // H storage scale
+    bool record_series;
+    bool use_graph;
+    hipStream_t stream = nullptr;
+    std::vector<void*> allocs;
+    // fields [parity][comp]
+    void* E[2][3] = {};
+    void* H[2][3] = {};
+    oafd::StepArgs<R> base{};
+    oafd::SibcArgs<R> sib{};
+    oafd::InjectArgs<R> inj{};
+    oafd::ProbeArgs prb{};
+    oafd::EnergyArgs ena{};
+    R* psiH[2] = {nullptr, nullptr};
+    long long* step_base = nullptr;
+    // probes / DFT
+    int nP[2] = {0, 0};
+    int nf = 0;
+    double* ring[2] = {nullptr, nullptr};
+    double* acc[2] = {nullptr, nullptr};
+    double* cmp[2] = {nullptr, nullptr};
+    double* omega = nullptr;
+    std::vector<double> host_series[2];
+    int64_t series_steps = 0;
+    double* partial = nullptr;
+    double* scalar = nullptr;
+    dim3 grid_yee;
+    bool have_sibc = false, have_inj = false, have_probe = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    int graph_state = 0;  // 0 not tried, 1 ready, -1 unavailable
+
+    template <typename T>
+    T* dev(size_t count, const T* host = nullptr) {
+        if (count == 0) count = 1;
+        void* p = nullptr;
+        HIP_TRY(hipMalloc(&p, count * sizeof(T)));
+        allocs.push_back(p);
+        cur_bytes += (int64_t)(count * sizeof(T));
+        peak = std::max(peak, cur_bytes);
+        if (host) {
+            HIP_TRY(hipMemcpy(p, host, count * sizeof(T), hipMemcpyHostToDevice));
+        } else {
+            HIP_TRY(hipMemset(p, 0, count * sizeof(T)));
+        }
+        return static_cast<T*>(p);
+    }
+    template <typename T>
+    T* dev_vec(const std::vector<T>& v) {
+        return dev<T>(v.size(), v.empty() ? nullptr : v.data());
+    }
+
+    ~Engine() override {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (void* p : allocs) (void)hipFree(p);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    void init(const oafd_desc* d, bool rec, int64_t maxs, bool graph_on) {
+        nx = d->nx; ny = d->ny; nz = d->nz;
+        X = d->X; Y = d->Y; Z = d->Z; B = d->batch;
+        bs = (long long)X * Y * Z;
+        dt = d->dt;
+        record_series = rec;
+        max_steps = maxs;
+        use_graph = graph_on;
+        constexpr bool half = sizeof(S) == 2;
+        hs = half ? ETA0 : 1.0;
+        HIP_TRY(hipStreamCreate(&stream));
+        const size_t nfield = (size_t)B * bs;
+        for (int p = 0; p < 2; ++p)
+            for (int c = 0; c < 3; ++c) {
+                E[p][c] = dev<S>(nfield);
+                H[p][c] = dev<S>(nfield);
+            }
+        auto& A = base;
+        A.nx = nx; A.ny = ny; A.nz = nz; A.X = X; A.Y = Y; A.Z = Z; A.bs = bs; A.B = B;
+        A.dtmu = (R)(d->dtmu * hs);
+        A.inv_hs = (R)(1.0 / hs);
+        uint16_t* idx[3];
+        for (int c = 0; c < 3; ++c) {
+            idx[c] = dev<uint16_t>((size_t)bs, d->idx[c]);
+            A.idx[c] = idx[c];
+        }
+        A.ca = dev_vec(cast_vec<R>(d->ca, d->n_mat));
+        A.cb = dev_vec(cast_vec<R>(d->cb, d->n_mat));
+        const int len[3] = {X, Y, Z};
+        for (int a = 0; a < 3; ++a) {
+            A.ip[a] = dev_vec(cast_vec<R>(d->inv_primal[a], len[a]));
+            A.id[a] = dev_vec(cast_vec<R>(d->inv_dual[a], len[a]));
+        }
+        for (int a = 0; a < 3; ++a) {
+            A.pmc_lo[a] = d->walls[2 * a] == 1;
+            A.pmc_hi[a] = d->walls[2 * a + 1] == 1;
+            for (int q = 0; q < 3; ++q) {
+                A.pe[a][q] = d->pml[a][q] ? dev_vec(cast_vec<R>(d->pml[a][q], len[a])) : nullptr;
+                A.ph[a][q] = d->pml[a][3 + q] ? dev_vec(cast_vec<R>(d->pml[a][3 + q], len[a])) : nullptr;
+            }
+        }
+        // CPML correction boxes: psi_E in place, psi_H double-buffered
+        std::memset(A.corrE, -1, sizeof(A.corrE));
+        std::memset(A.corrH, -1, sizeof(A.corrH));
+        long long offE = 0, offH = 0;
+        for (int n = 0; n < d->n_corr; ++n) {
+            const int32_t* r = d->corr + 12 * n;
+            const int kind = r[0], c = r[1], a = r[2], side = r[5];
+            const int t = (a == (c + 1) % 3) ? 0 : 1;
+            oafd::CorrBox& bx = A.box[n];
+            bx.lo[0] = r[6]; bx.n[0] = r[7] - r[6];
+            bx.lo[1] = r[8]; bx.n[1] = r[9] - r[8];
+            bx.lo[2] = r[10]; bx.n[2] = r[11] - r[10];
+            bx.vol = (long long)bx.n[0] * bx.n[1] * bx.n[2];
+            if (kind == 0) {
+                bx.off = offE;
+                offE += bx.vol * B;
+                A.corrE[c][t][side] = (signed char)n;
+            } else {
+                bx.off = offH;
+                offH += bx.vol * B;
+                A.corrH[c][t][side] = (signed char)n;
+            }
+        }
+        A.psiE = dev<R>((size_t)offE);
+        psiH[0] = dev<R>((size_t)offH);
+        psiH[1] = dev<R>((size_t)offH);
+        // Debye ADE
+        A.npole = d->n_poles;
+        A.ade_first = d->n_poles > 0 ? d->ade_first : d->n_mat + 1;
+        A.ade_nrow = d->n_mat - d->ade_first;
+        A.jk0 = d->ade_k0;
+        A.njk = d->ade_nk;
+        if (d->n_poles > 0) {
+            for (int p = 0; p < d->n_poles; ++p) {
+                A.kp[p] = (R)d->kp[p];
+                A.kq[p] = (R)(0.5 * (1.0 + d->kp[p]));
+            }
+            A.beta = dev_vec(cast_vec<R>(d->beta, (size_t)d->n_poles * A.ade_nrow));
+            A.J = dev<R>((size_t)d->n_poles * B * 3 * A.njk * Y * X);
+        }
+        // z chunking: enough workgroups to fill 256 CUs several times over
+        const int ntx = (nx + 1 + oafd::TXE - 1) / oafd::TXE;
+        const int tye = NWAVE_ROWS() - 1;
+        const int nty = (ny + 1 + tye - 1) / tye;
+        const long long tiles = (long long)ntx * nty * B;
+        int nkc = (int)std::max<long long>(1, std::min<long long>(nz + 1, (4096 + tiles - 1) / tiles));
+        int kchunk = (nz + 1 + nkc - 1) / nkc;
+        kchunk = std::max(kchunk, 4);
+        nkc = (nz + 1 + kchunk - 1) / kchunk;
+        A.kchunk = kchunk;
+        A.nkc = nkc;
+        grid_yee = dim3(ntx, nty, nkc * B);
+        // SIBC
+        if (d->sibc_n > 0) {
+            have_sibc = true;
+            sib.n = d->sibc_n; sib.np = d->sibc_np; sib.B = B; sib.bs = bs;
+            sib.inv_hs = (R)(1.0 / hs);
+            std::vector<int> comp(d->sibc_comp, d->sibc_comp + d->sibc_n), mat(d->sibc_mat, d->sibc_mat + d->sibc_n);
+            std::vector<int> flat(d->sibc_n), hcomp(2 * d->sibc_n), hflat(2 * d->sibc_n);
+            for (int e = 0; e < d->sibc_n; ++e) flat[e] = (int)d->sibc_flat[e];
+            for (int e = 0; e < 2 * d->sibc_n; ++e) {
+                hcomp[e] = d->sibc_hcomp[e];
+                hflat[e] = (int)d->sibc_hflat[e];
+            }
+            sib.comp = dev_vec(comp);
+            sib.flat = dev_vec(flat);
+            sib.hcomp = dev_vec(hcomp);
+            sib.hflat = dev_vec(hflat);
+            sib.hsign = dev_vec(cast_vec<R>(d->sibc_hsign, 2 * (size_t)d->sibc_n));
+            sib.mat = dev_vec(mat);
+            sib.r0 = dev_vec(cast_vec<R>(d->sibc_r0, d->sibc_nmat));
+            sib.alpha = dev_vec(cast_vec<R>(d->sibc_alpha, (size_t)d->sibc_nmat * d->sibc_np));
+            sib.gp = dev_vec(cast_vec<R>(d->sibc_gp, d->sibc_np));
+            sib.jprev = dev<R>((size_t)B * d->sibc_n);
+            sib.v = dev<R>((size_t)d->sibc_np * B * d->sibc_n);
+        }
+        step_base = dev<long long>(1);
+        // injection
+        if (d->inj_n > 0) {
+            have_inj = true;
+            inj.n = d->inj_n; inj.B = B; inj.nch = d->n_channels; inj.bs = bs;
+            inj.max_steps = maxs;
+            std::vector<int> comp(d->inj_comp, d->inj_comp + d->inj_n), flat(d->inj_n);
+            for (int t = 0; t < d->inj_n; ++t) flat[t] = (int)d->inj_flat[t];
+            std::vector<int> start(d->inj_start, d->inj_start + d->inj_n + 1);
+            std::vector<int> chan(d->inj_chan, d->inj_chan + d->inj_terms);
+            std::vector<double> amp(d->inj_amp, d->inj_amp + d->inj_terms);
+            std::vector<double> drive(d->drive, d->drive + (size_t)B * d->n_channels);
+            std::vector<double> wave(d->wave, d->wave + (size_t)d->n_channels * maxs);
+            for (int c = 0; c < 3; ++c) inj.idx[c] = idx[c];
+            inj.comp = dev_vec(comp);
+            inj.flat = dev_vec(flat);
+            inj.start = dev_vec(start);
+            inj.amp = dev_vec(amp);
+            inj.chan = dev_vec(chan);
+            inj.drive = dev_vec(drive);
+            inj.wave = dev_vec(wave);
+            inj.step_base = step_base;
+            inj.Y = Y; inj.X = X;
+            inj.npole = A.npole; inj.ade_first = A.ade_first; inj.ade_nrow = A.ade_nrow;
+            inj.jk0 = A.jk0; inj.njk = A.njk;
+            inj.beta = A.beta; inj.J = A.J;
+        }
+        // probes
+        nf = d->n_freq;
+        std::vector<double> om(nf);
+        for (int f = 0; f < nf; ++f) om[f] = 2.0 * 3.14159265358979323846 * d->freqs[f];
+        omega = dev_vec(om);
+        nP[0] = d->n_probe[0];
+        nP[1] = d->n_probe[1];
+        if (nP[0] + nP[1] > 0) {
+            have_probe = true;
+            std::vector<int> comp, flat;
+            std::vector<long long> off{0};
+            std::vector<double> w;
+            for (int kind = 0; kind < 2; ++kind)
+                for (int p = 0; p < nP[kind]; ++p) {
+                    comp.push_back(d->probe_comp[kind][p] + 3 * kind);
+                    for (int64_t e = d->probe_off[kind][p]; e < d->probe_off[kind][p + 1]; ++e) {
+                        flat.push_back((int)d->probe_flat[kind][e]);
+                        w.push_back(d->probe_w[kind][e]);
+                    }
+                    off.push_back((long long)flat.size());
+                }
+            prb.nE = nP[0]; prb.nH = nP[1]; prb.B = B; prb.bs = bs; prb.inv_hs = 1.0 / hs;
+            prb.comp = dev_vec(comp);
+            prb.off = dev_vec(off);
+            prb.flat = dev_vec(flat);
+            prb.w = dev_vec(w);
+            for (int kind = 0; kind < 2; ++kind) {
+                ring[kind] = dev<double>((size_t)CHUNK * B * std::max(nP[kind], 1));
+                acc[kind] = dev<double>((size_t)2 * nf * B * std::max(nP[kind], 1));
+                cmp[kind] = dev<double>((size_t)2 * nf * B * std::max(nP[kind], 1));
+            }
+            prb.ringE = ring[0];
+            prb.ringH = ring[1];
+        }
+        // energy
+        ena.X = X; ena.Y = Y; ena.Z = Z; ena.B = B; ena.bs = bs; ena.inv_hs = 1.0 / hs;
+        for (int c = 0; c < 3; ++c) ena.idx[c] = idx[c];
+        std::vector<double> eps(d->n_mat);
+        for (int m = 0; m < d->n_mat; ++m) eps[m] = d->eps[m] * EPS0;
+        ena.eps = dev_vec(eps);
+        for (int a = 0; a < 3; ++a) {
+            ena.nw[a] = dev<double>(len[a], d->node_w[a]);
+            ena.cw[a] = dev<double>(len[a], d->cell_w[a]);
+        }
+        ena.mu0 = MU0;
+        partial = dev<double>((size_t)B * Z * Y);
+        scalar = dev<double>(1);
+        ena.partial = partial;
+        HIP_TRY(hipDeviceSynchronize());
+    }
+
+    static constexpr int NWAVE_ROWS() { return oafd::NWAVE * RPWV; }
+
+    // ---- launches
+    void launch_yee(int parity) {
+        oafd::StepArgs<R> A = base;
+        for (int c = 0; c < 3; ++c) {
+            A.Ein[c] = E[parity][c];
+            A.Hin[c] = H[parity][c];
+            A.Eout[c] = E[parity ^ 1][c];
+            A.Hout[c] = H[parity ^ 1][c];
+        }
+        A.psiHin = psiH[parity];
+        A.psiHout = psiH[parity ^ 1];
+        hipLaunchKernelGGL((oafd::yee_step<S, R, RPWV>), grid_yee, dim3(oafd::TXH, oafd::NWAVE), 0, stream, A);
+        HIP_TRY(hipGetLastError());
+    }
+    void launch_rest(int slot, int parity) {
+        const int out = parity ^ 1;
+        if (have_sibc) {
+            oafd::SibcArgs<R> a = sib;
+            for (int c = 0; c < 3; ++c) {
+                a.E[c] = E[out][c];
+                a.H[c] = H[out][c];
+            }
+            hipLaunchKernelGGL((oafd::sibc_step<S, R>), dim3((a.n + 255) / 256, B), dim3(256), 0, stream, a);
+            HIP_TRY(hipGetLastError());
+        }
+        if (have_inj) {
+            oafd::InjectArgs<R> a = inj;
+            for (int c = 0; c < 3; ++c) a.E[c] = E[out][c];
+            a.step_off = slot;
+            hipLaunchKernelGGL((oafd::inject_step<S, R>), dim3((a.n + 255) / 256), dim3(256), 0, stream, a);
+            HIP_TRY(hipGetLastError());
+        }
+        if (have_probe) {
+            oafd::ProbeArgs a = prb;
+            for (int c = 0; c < 3; ++c) {
+                a.F[c] = E[out][c];
+                a.F[3 + c] = H[out][c];
+            }
+            a.slot = slot;
+            hipLaunchKernelGGL((oafd::probe_step<S>), dim3(nP[0] + nP[1], B), dim3(256), 0, stream, a);
+            HIP_TRY(hipGetLastError());
+        }
+    }
+    void launch_step(int slot, int parity) {
+        launch_yee(parity);
+        launch_rest(slot, parity);
+    }
+    void launch_dft(int slot0, int count) {
+        if (!have_probe) return;
+        for (int kind = 0; kind < 2; ++kind) {
+            if (nP[kind] == 0) continue;
+            const int BP = B * nP[kind];
+            const long long nthr = (long long)nf * BP;
+            hipLaunchKernelGGL(oafd::dft_accum, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, stream,
+                               ring[kind], BP, nf, omega, dt, kind == 0 ? 1.0 : 0.5, step_base, slot0, count,
+                               acc[kind], cmp[kind]);
+            HIP_TRY(hipGetLastError());
+        }
+    }
+    void launch_advance(int by) {
+        hipLaunchKernelGGL(oafd::advance_step, dim3(1), dim3(64), 0, stream, step_base, by);
+        HIP_TRY(hipGetLastError());
+    }
+    void capture() {
+        hipError_t e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) {
+            graph_state = -1;
+            (void)hipGetLastError();
+            return;
+        }
+        bool ok = true;
+        try {
+            for (int s = 0; s < CHUNK; ++s) launch_step(s, s & 1);
+            launch_dft(0, CHUNK);
+            launch_advance(CHUNK);
+        } catch (const HipFail&) {
+            ok = false;
+        }
+        hipGraph_t g = nullptr;
+        e = hipStreamEndCapture(stream, &g);
+        if (!ok || e != hipSuccess || g == nullptr) {
+            graph_state = -1;
+            if (g) (void)hipGraphDestroy(g);
+            (void)hipGetLastError();
+            return;
+        }
+        e = hipGraphInstantiate(&graph_exec, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            graph_state = -1;
+            (void)hipGraphDestroy(g);
+            (void)hipGetLastError();
+            return;
+        }
+        graph = g;
+        graph_state = 1;
+    }
+    void pull_series(int slot0, int count) {
+        if (!record_series) return;
+        HIP_TRY(hipStreamSynchronize(stream));
+        for (int kind = 0; kind < 2; ++kind) {
+            if (nP[kind] == 0) continue;
+            const size_t per = (size_t)B * nP[kind];
+            const size_t old = host_series[kind].size();
+            host_series[kind].resize(old + per * count);
+            HIP_TRY(hipMemcpy(host_series[kind].data() + old, ring[kind] + per * slot0, per * count * sizeof(double),
+                              hipMemcpyDeviceToHost));
+        }
+        series_steps += count;
+    }
+
+    int step(int64_t n) override {
+        if (n < 0) return fail(OAFD_E_INVALID, "negative step count");
+        if (step_count + n > max_steps) return fail(OAFD_E_MAX_STEPS, "step count exceeds max_steps (wave table length)");
+        int64_t done = 0;
+        while (done < n) {
+            const int64_t start = step_count;
+            const int in_chunk = (int)(start % CHUNK);
+            const int take = (int)std::min<int64_t>(n - done, CHUNK - in_chunk);
+            if (use_graph && take == CHUNK && graph_state == 0) capture();
+            if (graph_state == 1 && take == CHUNK) {
+                HIP_TRY(hipGraphLaunch(graph_exec, stream));
+            } else {
+                for (int s = 0; s < take; ++s) launch_step(in_chunk + s, (int)((start + s) & 1));
+                launch_dft(in_chunk, take);
+                if (in_chunk + take == CHUNK) launch_advance(CHUNK);
+            }
+            pull_series(in_chunk, take);
+            step_count += take;
+            done += take;
+        }
+        HIP_TRY(hipStreamSynchronize(stream));
+        return OAFD_OK;
+    }
+
+    int step_timed(int64_t n, double* out) override {
+        if (step_count + n > max_steps) return fail(OAFD_E_MAX_STEPS, "step count exceeds max_steps (wave table length)");
+        std::vector<hipEvent_t> ev(2 * n + 2);
+        for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
+        double yee_ms = 0.0, rest_ms = 0.0, wall_ms = 0.0;
+        int64_t done = 0;
+        HIP_TRY(hipEventRecord(ev[0], stream));
+        int64_t evi = 1;
+        std::vector<std::pair<int, int>> pairs;
+        while (done < n) {
+            const int64_t start = step_count;
+            const int in_chunk = (int)(start % CHUNK);
+            const int take = (int)std::min<int64_t>(n - done, CHUNK - in_chunk);
+            for (int s = 0; s < take; ++s) {
+                const int parity = (int)((start + s) & 1);
+                HIP_TRY(hipEventRecord(ev[evi], stream));
+                launch_yee(parity);
+                HIP_TRY(hipEventRecord(ev[evi + 1], stream));
+                pairs.emplace_back((int)evi, (int)evi + 1);
+                evi += 2;
+                launch_rest(in_chunk + s, parity);
+            }
+            launch_dft(in_chunk, take);
+            if (in_chunk + take == CHUNK) launch_advance(CHUNK);
+            pull_series(in_chunk, take);
+            step_count += take;
+            done += take;
+        }
+        HIP_TRY(hipEventRecord(ev[evi], stream));
+        HIP_TRY(hipEventSynchronize(ev[evi]));
+        float ms = 0.f;
+        for (auto& pr : pairs) {
+            HIP_TRY(hipEventElapsedTime(&ms, ev[pr.first], ev[pr.second]));
+            yee_ms += ms;
+        }
+        HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[evi]));
+        wall_ms = ms;
+        for (size_t q = 1; q < pairs.size(); ++q) {
+            HIP_TRY(hipEventElapsedTime(&ms, ev[pairs[q - 1].second], ev[pairs[q].first]));
+            rest_ms += ms;
+        }
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        out[0] = yee_ms;
+        out[1] = wall_ms;
+        out[2] = (double)pairs.size();
+        out[3] = rest_ms;
+        return OAFD_OK;
+    }
+
+    int energy(double* out) override {
+        oafd::EnergyArgs a = ena;
+        const int cur = (int)(step_count & 1);
+        for (int c = 0; c < 3; ++c) {
+            a.F[c] = E[cur][c];
+            a.F[3 + c] = H[cur][c];
+        }
+        hipLaunchKernelGGL((oafd::energy_rows<S>), dim3(Y, Z, B), dim3(256), 0, stream, a);
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(oafd::sum_partials, dim3(1), dim3(256), 0, stream, partial, (long long)B * Z * Y, scalar);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(out, scalar, sizeof(double), hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        return OAFD_OK;
+    }
+
+    int results(int kind, double* out) override {
+        if (kind < 0 || kind > 1) return fail(OAFD_E_INVALID, "kind must be 0 (E) or 1 (H)");
+        if (nP[kind] == 0) return OAFD_OK;
+        HIP_TRY(hipStreamSynchronize(stream));
+        HIP_TRY(hipMemcpy(out, acc[kind], sizeof(double) * 2 * nf * B * nP[kind], hipMemcpyDeviceToHost));
+        return OAFD_OK;
+    }
+
+    int series(int kind, double* out, int64_t cap) override {
+        if (kind < 0 || kind > 1) return fail(OAFD_E_INVALID, "kind must be 0 (E) or 1 (H)");
+        const auto& v = host_series[kind];
+        if ((int64_t)v.size() > cap) return fail(OAFD_E_INVALID, "series buffer too small");
+        std::memcpy(out, v.data(), v.size() * sizeof(double));
+        return OAFD_OK;
+    }
+
+    int get_field(int comp, double* out) override {
+        if (comp < 0 || comp > 5) return fail(OAFD_E_INVALID, "component must be 0..5");
+        const int cur = (int)(step_count & 1);
+        const void* src = comp < 3 ? E[cur][comp] : H[cur][comp - 3];
+        const long long n = (long long)B * bs;
+        double* tmp = nullptr;
+        HIP_TRY(hipMalloc(&tmp, n * sizeof(double)));
+        hipLaunchKernelGGL((oafd::to_f64<S>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, src, n,
+                           comp < 3 ? 1.0 : 1.0 / hs, tmp);
+        hipError_t e1 = hipGetLastError();
+        hipError_t e2 = hipMemcpyAsync(out, tmp, n * sizeof(double), hipMemcpyDeviceToHost, stream);
+        hipError_t e3 = hipStreamSynchronize(stream);
+        (void)hipFree(tmp);
+        HIP_TRY(e1);
+        HIP_TRY(e2);
+        HIP_TRY(e3);
+        return OAFD_OK;
+    }
+
+    int set_field(int comp, const double* in) override {
+        if (comp < 0 || comp > 5) return fail(OAFD_E_INVALID, "component must be 0..5");
+        const int cur = (int)(step_count & 1);
+        void* dst = comp < 3 ? E[cur][comp] : H[cur][comp - 3];
+        const long long n = (long long)B * bs;
+        double* tmp = nullptr;
+        HIP_TRY(hipMalloc(&tmp, n * sizeof(double)));
+        hipError_t e1 = hipMemcpyAsync(tmp, in, n * sizeof(double), hipMemcpyHostToDevice, stream);
+        hipLaunchKernelGGL((oafd::from_f64<S>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, tmp, n,
+                           comp < 3 ? 1.0 : hs, dst);
+        hipError_t e2 = hipGetLastError();
+        hipError_t e3 = hipStreamSynchronize(stream);
+        (void)hipFree(tmp);
+        HIP_TRY(e1);
+        HIP_TRY(e2);
+        HIP_TRY(e3);
+        return OAFD_OK;
+    }
+};
+
+int64_t estimate(const oafd_desc* d, int dtype) {
+    const int64_t s = dtype == OAFD_F64 ? 8 : (dtype == OAFD_F32 ? 4 : 2);
+    const int64_t r = dtype == OAFD_F64 ? 8 : 4;
+    const int64_t bs = (int64_t)d->X * d->Y * d->Z;
+    int64_t bytes = bs * (12 * s * d->batch + 6);  // double-buffered fields + 3 x u16 index
+    int64_t psiE = 0, psiH = 0;
+    for (int n = 0; n < d->n_corr; ++n) {
+        const int32_t* q = d->corr + 12 * n;
+        const int64_t v = (int64_t)(q[7] - q[6]) * (q[9] - q[8]) * (q[11] - q[10]) * d->batch;
+        (q[0] == 0 ? psiE : psiH) += v;
+    }
+    bytes += (psiE + 2 * psiH) * r;
+    if (d->n_poles > 0) bytes += (int64_t)d->n_poles * d->batch * 3 * d->ade_nk * d->Y * d->X * r;
+    if (d->sibc_n > 0) bytes += (int64_t)d->sibc_n * (d->batch * (1 + d->sibc_np) * r + 6 * 4 + 2 * r);
+    bytes += (int64_t)d->n_channels * 8 * 1;  // wave table counted by the caller's max_steps
+    return (int64_t)(bytes * 1.05);
+}
+
+int check_desc(const oafd_desc* d, int dtype) {
+    if (!d) return fail(OAFD_E_INVALID, "null descriptor");
+    if (dtype < OAFD_F32 || dtype > OAFD_BF16) return fail(OAFD_E_INVALID, "unknown dtype");
+    if (d->nx < 1 || d->ny < 1 || d->nz < 1 || d->batch < 1) return fail(OAFD_E_INVALID, "empty grid or batch");
+    if (d->X < d->nx + 2 || d->Y != d->ny + 2 || d->Z != d->nz + 2) return fail(OAFD_E_INVALID, "storage dims do not match the grid");
+    if ((int64_t)d->X * d->Y * d->Z >= (int64_t)1 << 31) return fail(OAFD_E_INVALID, "grid exceeds 2^31 cells per batch member");
+    if (d->n_poles < 0 || d->n_poles > OAFD_MAX_POLES) return fail(OAFD_E_INVALID, "too many Debye poles");
+    if (d->n_corr < 0 || d->n_corr > OAFD_MAX_CORR) return fail(OAFD_E_INVALID, "too many CPML corrections");
+    if (d->n_mat < 1 || d->n_mat > 65536) return fail(OAFD_E_INVALID, "bad material table size");
+    if (d->n_poles > 0 && (d->ade_first < 1 || d->ade_first > d->n_mat || d->ade_k0 < 0 || d->ade_k0 + d->ade_nk > d->Z))
+        return fail(OAFD_E_INVALID, "bad Debye table or plane range");
+    return OAFD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* oafd_version(void) { return "0.1.0"; }
+
+int oafd_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return n;
+}
+
+int oafd_device_info(int index, char* name, int name_len, char* arch, int arch_len, int64_t* total_bytes) {
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, index) != hipSuccess) return fail(OAFD_E_NO_DEVICE, "no such device");
+    if (name && name_len > 0) std::snprintf(name, name_len, "%s", p.name);
+    if (arch && arch_len > 0) std::snprintf(arch, arch_len, "%s", p.gcnArchName);
+    if (total_bytes) *total_bytes = (int64_t)p.totalGlobalMem;
+    return OAFD_OK;
+}
+
+int oafd_create(const oafd_desc* d, int dtype, int record_series, int64_t max_steps, int use_graph, oafd_engine** out) {
+    if (!out) return fail(OAFD_E_INVALID, "null output pointer");
+    *out = nullptr;
+    if (g_pending_create_fault) {
+        const int c = g_pending_create_fault;
+        g_pending_create_fault = 0;
+        return fail(c, "injected fault (oafd_debug_inject_fault)");
+    }
+    int rc = check_desc(d, dtype);
+    if (rc) return rc;
+    if (max_steps < 1) return fail(OAFD_E_INVALID, "max_steps must be positive");
+    if (oafd_device_count() < 1) return fail(OAFD_E_NO_DEVICE, "no ROCm device visible");
+    oafd_engine* e = nullptr;
+    try {
+        switch (dtype) {
+            case OAFD_F32: {
+                auto* x = new Engine<float, float, 4>();
+                e = x;
+                x->init(d, record_series != 0, max_steps, use_graph != 0);
+                break;
+            }
+            case OAFD_F64: {
+                auto* x = new Engine<double, double, 2>();
+                e = x;
+                x->init(d, record_series != 0, max_steps, use_graph != 0);
+                break;
+            }
+            case OAFD_F16: {
+                auto* x = new Engine<_Float16, float, 4>();
+                e = x;
+                x->init(d, record_series != 0, max_steps, use_graph != 0);
+                break;
+            }
+            default: {
+                auto* x = new Engine<__bf16, float, 4>();
+                e = x;
+                x->init(d, record_series != 0, max_steps, use_graph != 0);
+                break;
+            }
+        }
+    } catch (const HipFail& f) {
+        delete e;
+        return fail(f.code, f.msg);
+    } catch (const std::bad_alloc&) {
+        delete e;
+        return fail(OAFD_E_OOM, "host allocation failed");
+    }
+    *out = e;
+    return OAFD_OK;
+}
+
+#define ENGINE_CALL(e, body)                                                   \
+    do {                                                                       \
+        if (!(e)) return fail(OAFD_E_INVALID, "null engine");                  \
+        if ((e)->pending_fault) {                                              \
+            const int _c = (e)->pending_fault;                                 \
+            (e)->pending_fault = 0;                                            \
+            return fail(_c, "injected fault (oafd_debug_inject_fault)");       \
+        }                                                                      \
+        try {                                                                  \
+            body;                                                              \
+        } catch (const HipFail& f) {                                           \
+            (e)->err = f.msg;                                                  \
+            return fail(f.code, f.msg);                                        \
+        }                                                                      \
+    } while (0)
+
+int oafd_step(oafd_engine* e, int64_t n) { ENGINE_CALL(e, return e->step(n)); }
+int oafd_energy(oafd_engine* e, double* out) { ENGINE_CALL(e, return e->energy(out)); }
+int oafd_results(oafd_engine* e, int kind, double* out) { ENGINE_CALL(e, return e->results(kind, out)); }
+int oafd_series(oafd_engine* e, int kind, double* out, int64_t cap) { ENGINE_CALL(e, return e->series(kind, out, cap)); }
+int oafd_get_field(oafd_engine* e, int comp, double* out) { ENGINE_CALL(e, return e->get_field(comp, out)); }
+int oafd_set_field(oafd_engine* e, int comp, const double* in) { ENGINE_CALL(e, return e->set_field(comp, in)); }
+int oafd_step_timed(oafd_engine* e, int64_t n, double* out) { ENGINE_CALL(e, return e->step_timed(n, out)); }
+
+int64_t oafd_series_len(const oafd_engine* e) {
+    if (!e) return 0;
+    return static_cast<const oafd_engine*>(e)->step_count;  // series length equals steps when recorded
+}
+int64_t oafd_step_count(const oafd_engine* e) { return e ? e->step_count : 0; }
+int64_t oafd_peak_bytes(const oafd_engine* e) { return e ? e->peak : 0; }
+int64_t oafd_estimate_bytes(const oafd_desc* d, int dtype) {
+    if (check_desc(d, dtype)) return -1;
+    return estimate(d, dtype);
+}
+
+int oafd_debug_inject_fault(oafd_engine* e, int code) {
+    if (!e) {
+        g_pending_create_fault = code;
+        return OAFD_OK;
+    }
+    e->pending_fault = code;
+    return OAFD_OK;
+}
+
+const char* oafd_last_error(const oafd_engine* e) {
+    (void)e;
+    return g_err.c_str();
+}
+
+void oafd_destroy(oafd_engine* e) { delete e; }
+
+}  // extern "C"

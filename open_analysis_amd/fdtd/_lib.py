@@ -1,0 +1,123 @@
+"""ctypes binding of ``liboafdtd_hip.so`` (the C ABI declared in ``include/oafdtd.h``).
+
+The library is built in-tree by ``open_analysis_amd/csrc/build.py``. Loading fails loudly with
+``AccelError(ACCEL_NOT_BUILT)`` when it is missing: there is no fallback path.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parents[1] / "csrc" / "liboafdtd_hip.so"
+
+OAFD_OK, OAFD_E_INVALID, OAFD_E_OOM, OAFD_E_DEVICE, OAFD_E_MAX_STEPS, OAFD_E_NO_DEVICE = range(6)
+DTYPES = {"float32": 0, "float64": 1, "float16": 2, "bfloat16": 3}
+MAX_POLES = 16
+MAX_CORR = 24
+
+P = C.c_void_p
+
+
+class OafdDesc(C.Structure):
+    """Mirror of ``oafd_desc`` (include/oafdtd.h); field order and types must match."""
+
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+        ("X", C.c_int32), ("Y", C.c_int32), ("Z", C.c_int32),
+        ("batch", C.c_int32), ("n_mat", C.c_int32),
+        ("dt", C.c_double), ("dtmu", C.c_double),
+        ("idx", P * 3), ("ca", P), ("cb", P), ("eps", P),
+        ("n_poles", C.c_int32), ("ade_first", C.c_int32),
+        ("ade_k0", C.c_int32), ("ade_nk", C.c_int32),
+        ("kp", P), ("beta", P),
+        ("inv_primal", P * 3), ("inv_dual", P * 3), ("node_w", P * 3), ("cell_w", P * 3),
+        ("walls", C.c_int32 * 6),
+        ("pml", (P * 6) * 3),
+        ("n_corr", C.c_int32), ("corr", P),
+        ("sibc_n", C.c_int32), ("sibc_np", C.c_int32), ("sibc_nmat", C.c_int32),
+        ("sibc_comp", P), ("sibc_flat", P), ("sibc_hcomp", P), ("sibc_hflat", P),
+        ("sibc_hsign", P), ("sibc_mat", P), ("sibc_r0", P), ("sibc_alpha", P), ("sibc_gp", P),
+        ("inj_n", C.c_int32), ("inj_terms", C.c_int32), ("n_channels", C.c_int32),
+        ("inj_comp", P), ("inj_flat", P), ("inj_start", P), ("inj_amp", P), ("inj_chan", P),
+        ("drive", P), ("wave", P),
+        ("n_probe", C.c_int32 * 2),
+        ("probe_comp", P * 2), ("probe_off", P * 2), ("probe_flat", P * 2), ("probe_w", P * 2),
+        ("n_freq", C.c_int32), ("freqs", P),
+    ]
+
+
+_LIB: C.CDLL | None = None
+
+
+def _declare(lib: C.CDLL) -> None:
+    E = C.c_void_p
+    sig = {
+        "oafd_version": (C.c_char_p, []),
+        "oafd_device_count": (C.c_int, []),
+        "oafd_device_info": (C.c_int, [C.c_int, C.c_char_p, C.c_int, C.c_char_p, C.c_int,
+                                       C.POINTER(C.c_int64)]),
+        "oafd_create": (C.c_int, [C.POINTER(OafdDesc), C.c_int, C.c_int, C.c_int64, C.c_int,
+                                  C.POINTER(E)]),
+        "oafd_step": (C.c_int, [E, C.c_int64]),
+        "oafd_energy": (C.c_int, [E, C.POINTER(C.c_double)]),
+        "oafd_results": (C.c_int, [E, C.c_int, P]),
+        "oafd_series_len": (C.c_int64, [E]),
+        "oafd_series": (C.c_int, [E, C.c_int, P, C.c_int64]),
+        "oafd_get_field": (C.c_int, [E, C.c_int, P]),
+        "oafd_set_field": (C.c_int, [E, C.c_int, P]),
+        "oafd_step_count": (C.c_int64, [E]),
+        "oafd_peak_bytes": (C.c_int64, [E]),
+        "oafd_estimate_bytes": (C.c_int64, [C.POINTER(OafdDesc), C.c_int]),
+        "oafd_step_timed": (C.c_int, [E, C.c_int64, P]),
+        "oafd_debug_inject_fault": (C.c_int, [E, C.c_int]),
+        "oafd_last_error": (C.c_char_p, [E]),
+        "oafd_destroy": (None, [E]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+
+
+EXPORTS = (
+    "oafd_version", "oafd_device_count", "oafd_device_info", "oafd_create", "oafd_step",
+    "oafd_energy", "oafd_results", "oafd_series_len", "oafd_series", "oafd_get_field",
+    "oafd_set_field", "oafd_step_count", "oafd_peak_bytes", "oafd_estimate_bytes",
+    "oafd_step_timed", "oafd_debug_inject_fault", "oafd_last_error", "oafd_destroy",
+)
+
+
+def lib() -> C.CDLL:
+    """The loaded library; raises ``AccelError(ACCEL_NOT_BUILT)`` when it is not built."""
+    global _LIB
+    if _LIB is None:
+        if not LIB_PATH.exists():
+            from ..accel import AccelError, ErrorInfo
+
+            raise AccelError(ErrorInfo("ACCEL_NOT_BUILT", f"{LIB_PATH.name} is not built",
+                                       str(LIB_PATH),
+                                       "Run python -m open_analysis_amd.csrc.build (hipcc)."))
+        handle = C.CDLL(str(LIB_PATH))
+        _declare(handle)
+        _LIB = handle
+    return _LIB
+
+
+def check(code: int, what: str) -> None:
+    """Map a status code to the reference's exception types."""
+    if code == OAFD_OK:
+        return
+    msg = (lib().oafd_last_error(None) or b"").decode(errors="replace")
+    from ..accel import AccelError, ErrorInfo
+
+    if code == OAFD_E_MAX_STEPS:
+        raise ValueError(msg or "step count exceeds max_steps (wave table length)")
+    if code == OAFD_E_INVALID:
+        raise ValueError(f"{what}: {msg}")
+    if code == OAFD_E_OOM:
+        raise AccelError(ErrorInfo("ACCEL_OUT_OF_MEMORY", f"{what}: {msg}", what,
+                                   "Reduce the problem or raise OA_GPU_MEM_GB."))
+    if code == OAFD_E_NO_DEVICE:
+        raise AccelError(ErrorInfo("ACCEL_NO_DEVICE", f"{what}: {msg}", what))
+    raise AccelError(ErrorInfo("ACCEL_DEVICE_ERROR", f"{what}: {msg}", what))

@@ -1,0 +1,320 @@
+"""HIP engine: the drop-in for the reference ``CupyEngine`` (``packages/oa-fdtd/src/oa_fdtd/
+engine.py:67-506``), driving the gfx950 kernels of ``liboafdtd_hip.so`` through ctypes.
+
+Same constructor arguments and methods (``step``, ``energy``, ``results``, ``series``,
+``get_fields``, ``peak_bytes``; ``name = "hip"``). What differs is below the boundary:
+
+- one fused kernel per step does the H update, CPML-H, PMC ghosts, the E update, CPML-E and the
+  Debye ADE (the reference launches about 30 kernels per step);
+- the Debye parameters live in the material table: every distinct (material, pole strengths)
+  pair of a dispersive edge gets its own ``uint16`` row, so the kernel needs no edge list;
+- fields are double-buffered on the device; psi exists only inside the CPML boxes;
+- the running DFT is accumulated on the device in FP64 with Kahan compensation.
+
+``dtype`` is ``float32`` (the drop-in), ``float64`` (GPU oracle), or ``float16``/``bfloat16``
+(field storage in 16 bits, arithmetic in FP32, H stored scaled by eta0).
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from typing import Any
+
+import numpy as np
+import numpy.typing as npt
+
+from . import _lib
+from .grid import MU0
+from .model import Compiled
+
+FloatArray = npt.NDArray[np.float64]
+CHUNK = 64
+_WALL = {"pec": 0, "pmc": 1, "cpml": 2}
+
+
+def _itemsize(dtype: str | int) -> int:
+    if isinstance(dtype, int):
+        return dtype
+    return {"float32": 4, "float64": 8, "float16": 2, "bfloat16": 2}[dtype]
+
+
+def extended_tables(c: Compiled) -> dict[str, Any]:
+    """Material table with one extra row per distinct (row, pole-strength vector) of the
+    dispersive edges, the remapped indices, the beta table and the ADE plane range."""
+    idx = [i.copy() for i in c.idx]
+    n0 = c.eps_tab.size
+    ca, cb, eps = c.ca_tab.copy(), c.cb_tab.copy(), c.eps_tab.copy()
+    out: dict[str, Any] = {"idx": idx, "n_poles": 0, "ade_first": n0, "ade_k0": 0, "ade_nk": 0,
+                           "kp": np.zeros(1), "beta": np.zeros(1)}
+    if c.ade is None:
+        out.update(ca=ca, cb=cb, eps=eps, n_mat=n0)
+        return out
+    npole = c.ade.tau.size
+    flats = list(c.ade.flat)
+    rows = np.concatenate([idx[ci].ravel()[flat] for ci, flat in zip(c.ade.comp, flats,
+                                                                    strict=True)]).astype(np.int64)
+    beta_all = np.concatenate(c.ade.beta, axis=1)  # (P, n_edges)
+    # integer code of (row, beta_1..beta_P): per-pole value codes combined in mixed radix
+    code = rows.copy()
+    radix = int(n0)
+    vals = []
+    for p in range(npole):
+        u, q = np.unique(beta_all[p], return_inverse=True)
+        vals.append(u)
+        if radix * u.size >= 2**62:
+            raise ValueError("too many distinct dispersive edge parameters")
+        code += q.ravel().astype(np.int64) * radix
+        radix *= u.size
+    ucode, inv = np.unique(code, return_inverse=True)
+    inv = inv.ravel()
+    nrow = ucode.size
+    if n0 + nrow > 65535:
+        raise ValueError(f"{n0 + nrow} material rows exceed the uint16 table")
+    base = ucode % n0
+    rest = ucode // n0
+    beta_tab = np.zeros((npole, nrow))
+    for p in range(npole):
+        beta_tab[p] = vals[p][rest % vals[p].size]
+        rest //= vals[p].size
+    ca = np.concatenate([ca, c.ca_tab[base]])
+    cb = np.concatenate([cb, c.cb_tab[base]])
+    eps = np.concatenate([eps, c.eps_tab[base]])
+    off = 0
+    zs, ys, xs = c.grid.shape
+    kmin, kmax = zs, -1
+    for ci, flat in zip(c.ade.comp, flats, strict=True):
+        sel = inv[off : off + flat.size]
+        off += flat.size
+        idx[ci].ravel()[flat] = (n0 + sel).astype(np.uint16)
+        k = flat // (ys * xs)
+        kmin, kmax = min(kmin, int(k.min())), max(kmax, int(k.max()))
+    out.update(ca=ca, cb=cb, eps=eps, n_mat=n0 + nrow, n_poles=npole, ade_first=n0,
+               ade_k0=kmin, ade_nk=kmax - kmin + 1, kp=c.ade.kp.copy(), beta=beta_tab)
+    return out
+
+
+class _Keep:
+    """Holds the host arrays a descriptor points to until the C call has copied them."""
+
+    def __init__(self) -> None:
+        self.arrays: list[np.ndarray] = []
+
+    def ptr(self, a: Any, dtype: Any) -> int:
+        arr = np.ascontiguousarray(np.asarray(a, dtype=dtype))
+        if arr.size == 0:
+            arr = np.zeros(1, dtype=dtype)
+        self.arrays.append(arr)
+        return arr.ctypes.data
+
+
+def build_desc(c: Compiled, max_steps: int) -> tuple[_lib.OafdDesc, _Keep, dict[str, Any]]:
+    """Flatten a :class:`Compiled` into the ``oafd_desc`` the C ABI copies to the device."""
+    g = c.grid
+    keep = _Keep()
+    d = _lib.OafdDesc()
+    nx, ny, nz = g.n
+    d.nx, d.ny, d.nz = nx, ny, nz
+    d.Z, d.Y, d.X = g.shape
+    d.batch = c.batch
+    d.dt = c.dt
+    d.dtmu = c.dt / MU0
+    tabs = extended_tables(c)
+    for ci in range(3):
+        d.idx[ci] = keep.ptr(tabs["idx"][ci], np.uint16)
+    d.n_mat = tabs["n_mat"]
+    d.ca, d.cb, d.eps = (keep.ptr(tabs[k], np.float64) for k in ("ca", "cb", "eps"))
+    d.n_poles, d.ade_first = tabs["n_poles"], tabs["ade_first"]
+    d.ade_k0, d.ade_nk = tabs["ade_k0"], tabs["ade_nk"]
+    d.kp = keep.ptr(tabs["kp"], np.float64)
+    d.beta = keep.ptr(tabs["beta"], np.float64)
+    for a in range(3):
+        d.inv_primal[a] = keep.ptr(g.inv_primal(a), np.float64)
+        d.inv_dual[a] = keep.ptr(g.inv_dual(a), np.float64)
+        d.node_w[a] = keep.ptr(g.node_weights(a), np.float64)
+        d.cell_w[a] = keep.ptr(g.cell_weights(a), np.float64)
+    for s in range(6):
+        d.walls[s] = _WALL[c.walls[s]]
+    for a, co in c.pml.coeffs.items():
+        for q, name in enumerate(("eb", "ec", "ek", "hb", "hc", "hk")):
+            d.pml[a][q] = keep.ptr(getattr(co, name), np.float64)
+    corr = np.array([[0 if k.kind == "E" else 1, k.c, k.a, k.b, int(np.sign(k.sign)), k.side,
+                      k.box[0][0], k.box[0][1], k.box[1][0], k.box[1][1], k.box[2][0], k.box[2][1]]
+                     for k in c.pml.corrections], dtype=np.int32).reshape(-1, 12)
+    if corr.shape[0] > _lib.MAX_CORR:
+        raise ValueError("too many CPML corrections")
+    d.n_corr = corr.shape[0]
+    d.corr = keep.ptr(corr, np.int32)
+    sb = c.sibc
+    if sb is not None and sb.flat.size:
+        d.sibc_n, d.sibc_np, d.sibc_nmat = sb.flat.size, sb.cp.size, sb.r0.size
+        d.sibc_comp = keep.ptr(sb.comp, np.int32)
+        d.sibc_flat = keep.ptr(sb.flat, np.int64)
+        d.sibc_hcomp = keep.ptr(sb.hcomp, np.int32)
+        d.sibc_hflat = keep.ptr(sb.hflat, np.int64)
+        d.sibc_hsign = keep.ptr(sb.hsign, np.float64)
+        d.sibc_mat = keep.ptr(sb.mat, np.int32)
+        d.sibc_r0 = keep.ptr(sb.r0, np.float64)
+        d.sibc_alpha = keep.ptr(sb.rp * sb.cp[None, :], np.float64)
+        d.sibc_gp = keep.ptr(sb.gp, np.float64)
+    # sources: one target per distinct (component, edge), terms in channel order
+    nch = len(c.channels)
+    d.n_channels = nch
+    if nch:
+        comp = np.concatenate([np.full(ch["flat"].size, int(ch["comp"])) for ch in c.channels])
+        flat = np.concatenate([np.asarray(ch["flat"], np.int64) for ch in c.channels])
+        amp = np.concatenate([np.asarray(ch["amps"], np.float64) for ch in c.channels])
+        chan = np.concatenate([np.full(ch["flat"].size, ci) for ci, ch in enumerate(c.channels)])
+        key = comp.astype(np.int64) * (1 << 40) + flat
+        order = np.argsort(key, kind="stable")
+        key, comp, flat, amp, chan = key[order], comp[order], flat[order], amp[order], chan[order]
+        first = np.concatenate([[True], key[1:] != key[:-1]])
+        starts = np.concatenate([np.nonzero(first)[0], [key.size]]).astype(np.int32)
+        d.inj_n, d.inj_terms = int(first.sum()), key.size
+        d.inj_comp = keep.ptr(comp[first], np.int32)
+        d.inj_flat = keep.ptr(flat[first], np.int64)
+        d.inj_start = keep.ptr(starts, np.int32)
+        d.inj_amp = keep.ptr(amp, np.float64)
+        d.inj_chan = keep.ptr(chan, np.int32)
+        t = (np.arange(max_steps) + 0.5) * c.dt
+        wave = np.stack([ch["waveform"](t) for ch in c.channels])
+        d.wave = keep.ptr(wave, np.float64)
+    d.drive = keep.ptr(c.drive, np.float64)
+    kinds = {"E": [p for p in c.probes if p["kind"] == "E"],
+             "H": [p for p in c.probes if p["kind"] == "H"]}
+    for kk, kind in enumerate(("E", "H")):
+        plist = kinds[kind]
+        d.n_probe[kk] = len(plist)
+        if not plist:
+            continue
+        d.probe_comp[kk] = keep.ptr([p["comp"] for p in plist], np.int32)
+        off = np.concatenate([[0], np.cumsum([p["flat"].size for p in plist])])
+        d.probe_off[kk] = keep.ptr(off, np.int64)
+        d.probe_flat[kk] = keep.ptr(np.concatenate([p["flat"] for p in plist]), np.int64)
+        d.probe_w[kk] = keep.ptr(np.concatenate([p["weights"] for p in plist]), np.float64)
+    d.n_freq = c.frequencies.size
+    d.freqs = keep.ptr(c.frequencies, np.float64)
+    return d, keep, kinds
+
+
+def estimate_bytes(c: Compiled, dtype: str | int = "float32") -> int:
+    """Device bytes of the HIP engine (engine.py:29-40 counterpart): double-buffered fields
+    (12 x itemsize x B per cell), 6 B of index per cell, psi (E once, H twice), the Debye J
+    planes, SIBC state, +5 % for tables and buffers."""
+    s = _itemsize(dtype)
+    r = 8 if s == 8 else 4
+    b = c.batch
+    cells = int(np.prod(c.grid.shape))
+    psi_e = sum(int(np.prod(k.shape)) for k in c.pml.corrections if k.kind == "E") * b
+    psi_h = sum(int(np.prod(k.shape)) for k in c.pml.corrections if k.kind == "H") * b
+    total = cells * (12 * s * b + 6) + (psi_e + 2 * psi_h) * r
+    if c.ade is not None:
+        zs, ys, xs = c.grid.shape
+        ks = np.concatenate([f // (ys * xs) for f in c.ade.flat])
+        nk = int(ks.max() - ks.min() + 1)
+        total += c.ade.tau.size * b * 3 * nk * ys * xs * r
+    if c.sibc is not None:
+        total += c.sibc.flat.size * (b * (1 + c.sibc.cp.size) * r + 24 + 2 * r)
+    return int(total * 1.05)
+
+
+class HipEngine:
+    """gfx950 engine with the reference ``CupyEngine`` interface."""
+
+    name = "hip"
+
+    def __init__(self, c: Compiled, dtype: str = "float32", record_series: bool = False,
+                 xp: Any = None, max_steps: int = 200_000, use_graph: bool = True) -> None:
+        del xp  # the reference passes its array module; this engine owns its device memory
+        if dtype not in _lib.DTYPES:
+            raise ValueError(f"dtype {dtype!r} is not one of {sorted(_lib.DTYPES)}")
+        self.c = c
+        self.dtype = dtype
+        self.record_series = record_series
+        self.max_steps = int(max_steps)
+        self.use_graph = use_graph
+        self.B = c.batch
+        L = _lib.lib()
+        desc, keep, kinds = build_desc(c, self.max_steps)
+        self._probes = kinds
+        h = C.c_void_p()
+        _lib.check(L.oafd_create(C.byref(desc), _lib.DTYPES[dtype], int(record_series),
+                                 self.max_steps, int(use_graph), C.byref(h)), "oafd_create")
+        del keep
+        self._h = h
+        self._L = L
+
+    def __del__(self) -> None:
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._L.oafd_destroy(h)
+            self._h = None
+
+    @property
+    def step_count(self) -> int:
+        return int(self._L.oafd_step_count(self._h))
+
+    def step(self, n: int = 1) -> None:
+        _lib.check(self._L.oafd_step(self._h, int(n)), "oafd_step")
+
+    def step_timed(self, n: int) -> dict[str, float]:
+        out = np.zeros(4)
+        _lib.check(self._L.oafd_step_timed(self._h, int(n), out.ctypes.data), "oafd_step_timed")
+        return {"yee_ms": float(out[0]), "wall_ms": float(out[1]), "launches": float(out[2]),
+                "gap_ms": float(out[3])}
+
+    def energy(self) -> float:
+        v = C.c_double()
+        _lib.check(self._L.oafd_energy(self._h, C.byref(v)), "oafd_energy")
+        return float(v.value)
+
+    def results(self) -> dict[str, np.ndarray]:
+        out: dict[str, np.ndarray] = {}
+        nf = self.c.frequencies.size
+        for kk, kind in enumerate(("E", "H")):
+            plist = self._probes[kind]
+            if not plist:
+                continue
+            buf = np.zeros((nf, self.B, len(plist), 2))
+            _lib.check(self._L.oafd_results(self._h, kk, buf.ctypes.data), "oafd_results")
+            z = buf[..., 0] + 1j * buf[..., 1]
+            for pi, p in enumerate(plist):
+                out[p["name"]] = z[:, :, pi].T.copy()
+        return out
+
+    def series(self) -> dict[str, np.ndarray]:
+        out: dict[str, np.ndarray] = {}
+        if not self.record_series:
+            return out
+        steps = self.step_count
+        for kk, kind in enumerate(("E", "H")):
+            plist = self._probes[kind]
+            if not plist or steps == 0:
+                continue
+            buf = np.zeros((steps, self.B, len(plist)))
+            _lib.check(self._L.oafd_series(self._h, kk, buf.ctypes.data, buf.size), "oafd_series")
+            for pi, p in enumerate(plist):
+                out[p["name"]] = buf[:, :, pi].T.copy()
+        return out
+
+    def get_fields(self) -> dict[str, np.ndarray]:
+        shape = (self.B, *self.c.grid.shape)
+        out = {}
+        for comp, name in enumerate(("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")):
+            buf = np.zeros(shape)
+            _lib.check(self._L.oafd_get_field(self._h, comp, buf.ctypes.data), "oafd_get_field")
+            out[name] = buf
+        return out
+
+    def set_field(self, name: str, values: np.ndarray) -> None:
+        comp = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz").index(name)
+        buf = np.ascontiguousarray(np.broadcast_to(values, (self.B, *self.c.grid.shape)),
+                                   dtype=np.float64)
+        _lib.check(self._L.oafd_set_field(self._h, comp, buf.ctypes.data), "oafd_set_field")
+
+    def peak_bytes(self) -> int:
+        return int(self._L.oafd_peak_bytes(self._h))
+
+
+def hip_factory(c: Compiled, dtype: str, record_series: bool, max_steps: int):
+    """Engine factory for :func:`run` (``engine=hip_factory``) without the device policy."""
+    return HipEngine(c, dtype, record_series, max_steps=max_steps), {"backend": "rocm"}

@@ -1,0 +1,119 @@
+"""Synthetic problems of the benchmark workloads, as :class:`Model` builders.
+
+- ``vacuum_cube``: the reference's plain-kernel throughput problem (256^3 vacuum, 10-cell CPML;
+  packages/oa-fdtd/bench/throughput.py:23-31).
+- ``microstrip``: plain FP32 microstrip, 160x80x48 at 0.15 mm, 8-cell CPML (m=3, R0=1e-6),
+  eps_r 4.3 two-cell substrate, PEC ground and 4x130-cell strip, 50-ohm ports, 401 points
+  over 1-10 GHz.
+- ``dispersive_microstrip``: 400x200x100 at 0.1 mm, single-pole Debye substrate (eps_s 4.4,
+  eps_inf 4.0, tau 8 ps), SIBC copper (5.8e7 S/m) strip and ground, 10-cell CPML, 801 points over
+  1-20 GHz.
+- ``stripline_stack``: 800x400x200 six-layer stack-up with a coupled stripline pair; layer
+  permittivities drawn uniformly from 3.0-4.6 with a seeded generator, each layer a single-pole
+  Debye medium (delta 0.4, tau 8 ps), SIBC copper ground planes and strips, CPML on all faces.
+
+Every builder scales to smaller grids with the same structure (used by the parity tests).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from .grid import Grid
+from .model import Conductor, LumpedPort, Model, Region, Waveform
+from .pml import CpmlSpec
+
+DEBYE_POLE = ((0.4, 8e-12),)
+
+
+def vacuum_cube(n: int = 256, h: float = 1e-3, npml: int = 10) -> Model:
+    lines = np.arange(n + 1) * h
+    m = Model(Grid(lines, lines, lines), Waveform("gauss_mod", 5e9, 5e9), np.array([5e9]),
+              walls=("cpml",) * 6)
+    m.cpml = CpmlSpec(cells=npml)
+    return m
+
+
+def microstrip(nx: int = 160, ny: int = 80, nz: int = 48, d: float = 0.15e-3, npml: int = 8,
+               sub: int = 2, strip_w: int = 4, strip_len: int = 130, eps_r: float = 4.3,
+               freqs: np.ndarray | None = None) -> Model:
+    g = Grid(np.arange(nx + 1) * d, np.arange(ny + 1) * d, np.arange(nz + 1) * d)
+    f = np.linspace(1e9, 10e9, 401) if freqs is None else np.asarray(freqs)
+    m = Model(g, Waveform("gauss_mod", 6e9, 12e9), f, walls=("cpml",) * 6, courant=0.99)
+    m.cpml = CpmlSpec(cells=npml, m=3.0, r0=1e-6)
+    zg = (npml + 2) * d
+    zs = zg + sub * d
+    m.regions.append(Region((0, 0, zg), (nx * d, ny * d, zs), eps_r))
+    m.regions.append(Region((0, 0, zg), (nx * d, ny * d, zg), pec=True))
+    x0 = ((nx - strip_len) // 2) * d
+    x1 = x0 + strip_len * d
+    y0 = ((ny - strip_w) // 2) * d
+    y1 = y0 + strip_w * d
+    m.regions.append(Region((x0, y0, zs), (x1, y1, zs), pec=True))
+    m.ports.append(LumpedPort("p1", 2, (x0, y0, zg), (x0, y1, zs), 50.0))
+    m.ports.append(LumpedPort("p2", 2, (x1, y0, zg), (x1, y1, zs), 50.0))
+    return m
+
+
+def dispersive_microstrip(nx: int = 400, ny: int = 200, nz: int = 100, d: float = 0.1e-3,
+                          npml: int = 10, sub: int = 5, strip_w: int = 10,
+                          margin: int = 20, freqs: np.ndarray | None = None,
+                          t_cu: float = 35e-6) -> Model:
+    """Microstrip on a Debye substrate with SIBC copper: ground is a one-cell copper slab over
+    the whole x-y extent, the strip a one-cell copper bar ``strip_w`` cells wide running between
+    the ports ``margin`` cells inside the CPML."""
+    g = Grid(np.arange(nx + 1) * d, np.arange(ny + 1) * d, np.arange(nz + 1) * d)
+    f = np.linspace(1e9, 20e9, 801) if freqs is None else np.asarray(freqs)
+    m = Model(g, Waveform("gauss_mod", 10.5e9, 21e9), f, walls=("cpml",) * 6, courant=0.99)
+    m.cpml = CpmlSpec(cells=npml, m=3.0, r0=1e-6)
+    zg = (npml + 3) * d
+    zs = zg + sub * d
+    m.regions.append(Region((-1.0, -1.0, zg), (1.0, 1.0, zs), 4.0, poles=DEBYE_POLE))
+    m.conductors.append(Conductor((-1.0, -1.0, zg - d), (1.0, 1.0, zg), 5.8e7, t_cu))
+    x0 = (npml + margin) * d
+    x1 = (nx - npml - margin) * d
+    y0 = ((ny - strip_w) // 2) * d
+    y1 = y0 + strip_w * d
+    m.conductors.append(Conductor((x0, y0, zs), (x1, y1, zs + d), 5.8e7, t_cu))
+    yc = 0.5 * (y0 + y1)
+    m.ports.append(LumpedPort("p1", 2, (x0, yc, zg), (x0, yc, zs), 50.0))
+    m.ports.append(LumpedPort("p2", 2, (x1, yc, zg), (x1, yc, zs), 50.0))
+    return m
+
+
+def stripline_stack(nx: int = 800, ny: int = 400, nz: int = 200, d: float = 0.05e-3,
+                    npml: int = 10, seed: int = 7, strip_w: int = 6, gap: int = 6,
+                    margin: int = 20, freqs: np.ndarray | None = None,
+                    t_cu: float = 17e-6) -> tuple[Model, np.ndarray]:
+    """Six dielectric layers fill the z extent between the CPML slabs; copper ground planes
+    (one-cell SIBC slabs over the full x-y extent) sit on the interfaces 1|2 and 4|5, the coupled
+    pair on interface 2|3 (edge-coupled, ``gap`` cells apart). Four 50-ohm z-directed ports join
+    the strip ends to the lower ground plane. Returns the model and the layer permittivities."""
+    g = Grid(np.arange(nx + 1) * d, np.arange(ny + 1) * d, np.arange(nz + 1) * d)
+    f = np.linspace(1e9, 20e9, 801) if freqs is None else np.asarray(freqs)
+    m = Model(g, Waveform("gauss_mod", 10.5e9, 21e9), f, walls=("cpml",) * 6, courant=0.99)
+    m.cpml = CpmlSpec(cells=npml, m=3.0, r0=1e-6)
+    rng = np.random.default_rng(seed)
+    eps = rng.uniform(3.0, 4.6, size=6)
+    inner = nz - 2 * npml
+    bounds = npml + np.round(np.linspace(0, inner, 7)).astype(int)
+    for li in range(6):
+        z0, z1 = bounds[li] * d, bounds[li + 1] * d
+        m.regions.append(Region((-1.0, -1.0, z0), (1.0, 1.0, z1), float(eps[li]),
+                                poles=DEBYE_POLE))
+    zg1, zg2 = bounds[1] * d, bounds[4] * d
+    m.conductors.append(Conductor((-1.0, -1.0, zg1 - d), (1.0, 1.0, zg1), 5.8e7, t_cu))
+    m.conductors.append(Conductor((-1.0, -1.0, zg2), (1.0, 1.0, zg2 + d), 5.8e7, t_cu))
+    zs = bounds[2] * d
+    x0 = (npml + margin) * d
+    x1 = (nx - npml - margin) * d
+    yc = ny // 2
+    ya = (yc - gap // 2 - strip_w) * d, (yc - gap // 2) * d
+    yb = (yc + gap // 2) * d, (yc + gap // 2 + strip_w) * d
+    for name, (y0, y1) in (("a", ya), ("b", yb)):
+        m.conductors.append(Conductor((x0, y0, zs), (x1, y1, zs + d), 5.8e7, t_cu))
+        ym = 0.5 * (y0 + y1)
+        m.ports.append(LumpedPort(f"{name}1", 2, (x0, ym, zg1), (x0, ym, zs), 50.0))
+        m.ports.append(LumpedPort(f"{name}2", 2, (x1, ym, zg1), (x1, ym, zs), 50.0))
+    m.excite = ("a1",)
+    return m, eps

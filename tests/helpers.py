@@ -1,0 +1,113 @@
+"""Problem builders for the closed-form tests (the reference's tests/fdtd_helpers.py and
+tests/fdtd_phys_helpers.py describe the same physical set-ups)."""
+
+from __future__ import annotations
+
+import numpy as np
+
+from open_analysis_amd.fdtd import (
+    C0,
+    EPS0,
+    ETA0,
+    MU0,
+    Conductor,
+    CpmlSpec,
+    Grid,
+    LumpedPort,
+    Model,
+    Probe,
+    Region,
+    SoftSource,
+    Waveform,
+)
+from open_analysis_amd.fdtd.engine import hip_factory
+from oracle import oracle_factory
+
+ENGINES = {"oracle": oracle_factory, "hip": hip_factory}
+
+
+def engine_dtype(engine: str) -> str:
+    return "float64" if engine == "oracle" else "float32"
+
+
+def line_model(ny: int, d: float, freqs: np.ndarray, src_y: int, probe_y: int, cells: int = 10,
+               dc: bool = True) -> Model:
+    """Uniform TEM line: PMC x, PEC z, CPML along y; E_z plane source and 'mean' probe."""
+    lines = np.arange(3) * d
+    g = Grid(lines, np.arange(ny + 1) * d, lines)
+    wf = Waveform("gauss", bandwidth=45e9) if dc else Waveform("gauss_mod", 20e9, 40e9)
+    m = Model(g, wf, freqs, walls=("pmc", "pmc", "cpml", "cpml", "pec", "pec"),
+              cpml=CpmlSpec(cells=cells))
+    m.sources.append(SoftSource("s", 2, (0.0, src_y * d, 0.0), (2 * d, src_y * d, 2 * d)))
+    m.probes.append(Probe("p", "E", 2, (0.0, probe_y * d, 0.0), (2 * d, probe_y * d, 2 * d),
+                          "mean"))
+    return m
+
+
+def tem_model(f_max: float = 3e9, r_scale: float = 1.0) -> Model:
+    """Parallel-plate line 8 mm x 4 mm, 4x120x4 cells, PMC sides, lumped ports of Z0 = eta h/w."""
+    w, h = 0.008, 0.004
+    z0 = ETA0 * h / w
+    nx, nz, ny, dy = 4, 4, 120, 0.5e-3
+    g = Grid(np.linspace(0, w, nx + 1), np.arange(ny + 1) * dy, np.linspace(0, h, nz + 1))
+    ports = [LumpedPort("p1", 2, (0, 0.0, 0), (w, 0.0, h), z0 * r_scale),
+             LumpedPort("p2", 2, (0, ny * dy, 0), (w, ny * dy, h), z0)]
+    return Model(g, Waveform("gauss_mod", f_max / 2.0, f_max), np.linspace(0.5e9, f_max, 12),
+                 walls=("pmc", "pmc", "pmc", "pmc", "pec", "pec"), ports=ports)
+
+
+W_PLATE, H_PLATE = 0.008, 0.001
+
+
+def plate_line(length: float, dy: float, freqs: np.ndarray, f_max: float, *,
+               poles: tuple = (), eps_inf: float = 1.0, sigma_cu: float | None = None,
+               t_cu: float = 0.25e-3) -> Model:
+    """TEM line between two plates (one-cell boxes: SIBC copper or PEC), PMC side walls, lumped
+    ports across the gap at both ends."""
+    w, h = W_PLATE, H_PLATE
+    nx, nzg = 4, 4
+    dz = h / nzg
+    ny = round(length / dy)
+    zl = [0.0, t_cu, *[t_cu + (i + 1) * dz for i in range(nzg)], 2 * t_cu + h]
+    g = Grid(np.linspace(0, w, nx + 1), np.arange(ny + 1) * dy, np.array(zl))
+    z0 = ETA0 * h / w / np.sqrt(eps_inf)
+    ports = [LumpedPort("p1", 2, (0, 0, t_cu), (w, 0, t_cu + h), z0),
+             LumpedPort("p2", 2, (0, ny * dy, t_cu), (w, ny * dy, t_cu + h), z0)]
+    regions = []
+    if poles or eps_inf != 1.0:
+        regions = [Region((0, 0, t_cu), (w, length, t_cu + h), eps_inf, poles=poles)]
+    kind = "pec" if sigma_cu is None else "thick_sibc"
+    sig = 5.8e7 if sigma_cu is None else sigma_cu
+    conductors = [
+        Conductor((-1, -1, -1), (w + 1, length + 1, t_cu), sig, t_cu, model=kind),
+        Conductor((-1, -1, t_cu + h), (w + 1, length + 1, 3 * t_cu + h), sig, t_cu, model=kind),
+    ]
+    return Model(g, Waveform("gauss_mod", f_max / 2.0, f_max), freqs, regions=regions,
+                 conductors=conductors, walls=("pmc", "pmc", "pmc", "pmc", "pec", "pec"),
+                 ports=ports)
+
+
+def abcd_s(gamma, zc, length: float, r1: float, r2: float):
+    """(S11, S21) of a uniform line section between resistive ports (textbook ABCD cascade)."""
+    a = np.cosh(gamma * length)
+    b = zc * np.sinh(gamma * length)
+    c = np.sinh(gamma * length) / zc
+    den = a * r2 + b + c * r1 * r2 + a * r1
+    return (a * r2 + b - c * r1 * r2 - a * r1) / den, 2.0 * np.sqrt(r1 * r2) / den
+
+
+def dielectric_line_s(freqs, eps, length: float, r_port: float):
+    w = 2 * np.pi * freqs
+    root = np.sqrt(eps)
+    return abcd_s(1j * w * root / C0, ETA0 * H_PLATE / W_PLATE / root, length, r_port, r_port)
+
+
+def copper_line_s(freqs, zs, length: float, r_port: float):
+    w = 2 * np.pi * freqs
+    z_series = 1j * w * MU0 * H_PLATE / W_PLATE + 2.0 * zs / W_PLATE
+    y_shunt = 1j * w * EPS0 * W_PLATE / H_PLATE
+    return abcd_s(np.sqrt(z_series * y_shunt), np.sqrt(z_series / y_shunt), length, r_port, r_port)
+
+
+def db(x) -> np.ndarray:
+    return np.asarray(20 * np.log10(np.abs(x)))

@@ -1,0 +1,137 @@
+"""The C ABI library on the CPU: it loads, exports every symbol the header declares, refuses bad
+descriptors, and the fault hook reaches the Python error mapping (no GPU needed)."""
+
+from __future__ import annotations
+
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from open_analysis_amd.accel import AccelError
+from open_analysis_amd.fdtd import _lib
+from open_analysis_amd.fdtd.engine import HipEngine, build_desc, estimate_bytes, extended_tables
+from open_analysis_amd.fdtd.model import compile_model
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def _have_lib() -> bool:
+    return _lib.LIB_PATH.exists()
+
+
+needs_lib = pytest.mark.skipif(not _have_lib(), reason="liboafdtd_hip.so not built")
+
+
+@needs_lib
+def test_library_exports_every_declared_symbol() -> None:
+    header = (ROOT / "include" / "oafdtd.h").read_text()
+    declared = set(re.findall(r"\b(oafd_[a-z_]+)\s*\(", header))
+    L = _lib.lib()
+    for name in declared:
+        assert hasattr(L, name), name
+    assert declared == set(_lib.EXPORTS)
+    assert L.oafd_version().decode()
+
+
+@needs_lib
+def test_struct_layout_matches_header() -> None:
+    """The ctypes mirror has every field of the C struct, in order."""
+    header = (ROOT / "include" / "oafdtd.h").read_text()
+    body = header[header.index("typedef struct oafd_desc {"): header.index("} oafd_desc;")]
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    names = []
+    for decl in body.split(";"):
+        decl = decl.strip()
+        if not decl or decl.startswith("typedef"):
+            decl = decl.split("{", 1)[-1].strip()
+            if not decl:
+                continue
+        decl = re.sub(r"^(const\s+)?[a-z0-9_]+\s*\*?", "", decl).strip()
+        for part in decl.split(","):
+            part = part.strip().lstrip("*").strip()
+            if part:
+                names.append(part.split("[")[0].strip())
+    assert names == [f[0] for f in _lib.OafdDesc._fields_]
+
+
+@needs_lib
+def test_injected_create_fault_maps_to_accel_error() -> None:
+    import cases
+    from golden_io import OWN_CLASSES
+
+    c = compile_model(cases.cpml_box(OWN_CLASSES))
+    L = _lib.lib()
+    L.oafd_debug_inject_fault(None, _lib.OAFD_E_OOM)
+    with pytest.raises(AccelError) as ei:
+        HipEngine(c, "float32", max_steps=10)
+    assert ei.value.info.code == "ACCEL_OUT_OF_MEMORY"
+    L.oafd_debug_inject_fault(None, _lib.OAFD_E_DEVICE)
+    with pytest.raises(AccelError) as ei:
+        HipEngine(c, "float32", max_steps=10)
+    assert ei.value.info.code == "ACCEL_DEVICE_ERROR"
+
+
+@needs_lib
+def test_bad_descriptor_is_rejected() -> None:
+    import cases
+    from golden_io import OWN_CLASSES
+
+    c = compile_model(cases.cpml_box(OWN_CLASSES))
+    d, keep, _ = build_desc(c, 10)
+    d.Y = d.Y + 1
+    h = C.c_void_p()
+    rc = _lib.lib().oafd_create(C.byref(d), 0, 0, 10, 0, C.byref(h))
+    assert rc == _lib.OAFD_E_INVALID
+    assert not h.value
+    del keep
+
+
+@needs_lib
+def test_estimate_bytes_agrees_with_library() -> None:
+    import cases
+    from golden_io import OWN_CLASSES
+
+    for builder in (cases.gpu_cpu_problem, cases.debye_sibc_line, cases.cpml_box):
+        c = compile_model(builder(OWN_CLASSES))
+        d, keep, _ = build_desc(c, 10)
+        lib_est = _lib.lib().oafd_estimate_bytes(C.byref(d), 0)
+        py_est = estimate_bytes(c, "float32")
+        assert abs(lib_est - py_est) <= 0.02 * py_est + 4096
+        del keep
+
+
+def test_estimate_bytes_per_cell() -> None:
+    """Double-buffered FP32 fields + 6 B index: 54 B per cell plus psi (engine.py:29-40 counts
+    30-40 B for its single-buffered fields)."""
+    import cases
+    from golden_io import OWN_CLASSES
+
+    c = compile_model(cases.gpu_cpu_problem(OWN_CLASSES, 1))
+    cells = int(np.prod(c.grid.shape))
+    est = estimate_bytes(c, "float32")
+    assert cells * 54 < est < cells * 70
+
+
+def test_extended_table_keeps_coefficients() -> None:
+    """Dispersive rows copy ca/cb/eps of their base row and carry beta; the plain rows and the
+    non-dispersive indices are unchanged."""
+    import cases
+    from golden_io import OWN_CLASSES
+
+    c = compile_model(cases.debye_sibc_line(OWN_CLASSES))
+    t = extended_tables(c)
+    n0 = c.eps_tab.size
+    assert t["n_mat"] > n0 and t["ade_first"] == n0
+    for ci, flat, beta in zip(c.ade.comp, c.ade.flat, c.ade.beta, strict=True):
+        rows = t["idx"][ci].ravel()[flat].astype(int)
+        assert (rows >= n0).all()
+        base = c.idx[ci].ravel()[flat]
+        np.testing.assert_array_equal(t["ca"][rows], c.ca_tab[base])
+        np.testing.assert_array_equal(t["cb"][rows], c.cb_tab[base])
+        np.testing.assert_array_equal(t["beta"][:, rows - n0], beta)
+    for ci in range(3):
+        keep = t["idx"][ci] < n0
+        np.testing.assert_array_equal(t["idx"][ci][keep], c.idx[ci][keep])

@@ -1,0 +1,156 @@
+"""HIP engine vs the CPU oracle and the reference's golden vectors (GPU).
+
+Tolerances are the reference's own (packages/oa-fdtd/tests/test_gpu_cpu.py:71-83): probe DFTs
+within 1e-4 x max (float32) and 1e-10 x max (float64) of the float64 oracle. Fields after N steps
+are checked the same way, per field kind.
+"""
+
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+from golden_io import build, field_errors, load
+
+from open_analysis_amd.fdtd import compile_model, run
+from open_analysis_amd.fdtd.engine import HipEngine
+from oracle import OracleEngine, oracle_factory
+
+pytestmark = pytest.mark.gpu
+TOL = {"float32": 1e-4, "float64": 1e-10}
+
+
+def _say(msg: str) -> None:
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _probe_err(a: dict, b: dict) -> dict[str, float]:
+    out = {}
+    for k in b:
+        scale = float(np.abs(b[k]).max())
+        out[k] = float(np.abs(a[k] - b[k]).max()) / max(scale, 1e-300)
+    return out
+
+
+def _fields_err(a: dict, b: dict) -> dict[str, float]:
+    out = {}
+    for kind in ("E", "H"):
+        names = [kind + x for x in "xyz"]
+        scale = max(float(np.abs(b[n]).max()) for n in names)
+        for n in names:
+            out[n] = float(np.abs(a[n] - b[n]).max()) / max(scale, 1e-300)
+    return out
+
+
+@pytest.mark.parametrize("name", ["gpu_cpu", "debye_sibc_line", "cpml_box", "microstrip_small"])
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+def test_hip_matches_golden(name: str, dtype: str) -> None:
+    """HIP fields and probe DFTs after N steps vs the reference NumpyEngine vectors."""
+    _say(f"[golden] case={name} dtype={dtype}")
+    gold = load(name)
+    model, steps = build(name)
+    c = compile_model(model)
+    eng = HipEngine(c, dtype, max_steps=steps + 10)
+    eng.step(steps)
+    err = field_errors(eng.get_fields(), gold)
+    assert max(err.values()) < TOL[dtype], err
+    for k, v in eng.results().items():
+        ref = gold["dft_" + k.replace(":", "_")]
+        assert np.abs(v - ref).max() <= TOL[dtype] * np.abs(ref).max(), k
+    assert eng.energy() == pytest.approx(float(gold["energy"]), rel=10 * TOL[dtype])
+
+
+@pytest.mark.parametrize("batch", [1, 2])
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+def test_hip_matches_oracle_run(batch: int, dtype: str) -> None:
+    """The reference's test_cuda_matches_numpy on the HIP engine (400 steps, run())."""
+    import cases
+    from golden_io import OWN_CLASSES
+
+    _say(f"[oracle-run] batch={batch} dtype={dtype}")
+    m = cases.gpu_cpu_problem(OWN_CLASSES, batch)
+    ref = run(m, engine=oracle_factory, dtype="float64", end_db=None, max_steps=400)
+    gpu = run(m, engine="hip", dtype=dtype, end_db=None, max_steps=400)
+    assert gpu.engine == "hip" and gpu.peak_bytes > 0
+    assert gpu.report["backend"] == "rocm"
+    err = _probe_err(gpu.probes, ref.probes)
+    assert max(err.values()) <= TOL[dtype], err
+
+
+def test_fields_match_oracle_with_end_criterion() -> None:
+    """Fields and energy after 300 steps on the graded CPML/PMC/PEC problem, batch 2."""
+    import cases
+    from golden_io import OWN_CLASSES
+
+    _say("[fields] gpu_cpu batch=2 float32 300 steps")
+    c = compile_model(cases.gpu_cpu_problem(OWN_CLASSES, 2))
+    a = HipEngine(c, "float32", max_steps=400)
+    b = OracleEngine(c, "float64")
+    a.step(300)
+    b.step(300)
+    assert max(_fields_err(a.get_fields(), b.get_fields()).values()) < 1e-4
+    assert a.energy() == pytest.approx(b.energy(), rel=1e-4)
+
+
+def test_graph_equals_eager() -> None:
+    """Same device, same input: the 64-step graph and eager launches are bitwise identical
+    (test_gpu_cpu.py:86-96)."""
+    import cases
+    from golden_io import OWN_CLASSES
+
+    _say("[graph-vs-eager]")
+    c = compile_model(cases.gpu_cpu_problem(OWN_CLASSES, 1))
+    a = HipEngine(c, "float32", use_graph=True, max_steps=700)
+    b = HipEngine(c, "float32", use_graph=False, max_steps=700)
+    a.step(256)
+    b.step(256)
+    ra, rb = a.results(), b.results()
+    for k, v in ra.items():
+        assert np.array_equal(v, rb[k]), k
+    fa, fb = a.get_fields(), b.get_fields()
+    for k in fa:
+        assert np.array_equal(fa[k], fb[k]), k
+
+
+def test_deterministic() -> None:
+    """Two runs of the same problem give bitwise identical results (test_gpu_cpu.py:99-105)."""
+    import cases
+    from golden_io import OWN_CLASSES
+
+    _say("[determinism]")
+    m = cases.gpu_cpu_problem(OWN_CLASSES, 1)
+    a = run(m, engine="hip", dtype="float32", end_db=None, max_steps=300)
+    b = run(m, engine="hip", dtype="float32", end_db=None, max_steps=300)
+    for k in a.probes:
+        assert np.array_equal(a.probes[k], b.probes[k])
+
+
+def test_series_matches_oracle() -> None:
+    import cases
+    from golden_io import OWN_CLASSES
+
+    _say("[series]")
+    c = compile_model(cases.gpu_cpu_problem(OWN_CLASSES, 2))
+    a = HipEngine(c, "float32", record_series=True, max_steps=200)
+    b = OracleEngine(c, "float64", record_series=True)
+    a.step(37)
+    a.step(100)
+    b.step(137)
+    sa, sb = a.series(), b.series()
+    assert set(sa) == set(sb)
+    for k in sb:
+        assert sa[k].shape == sb[k].shape
+        assert np.abs(sa[k] - sb[k]).max() <= 1e-4 * np.abs(sb[k]).max(), k
+
+
+def test_max_steps_is_enforced() -> None:
+    import cases
+    from golden_io import OWN_CLASSES
+
+    c = compile_model(cases.cpml_box(OWN_CLASSES))
+    e = HipEngine(c, "float32", max_steps=10)
+    e.step(10)
+    with pytest.raises(ValueError):
+        e.step(1)
