@@ -136,6 +136,7 @@ def main() -> int:
     yee_avg_s = tm["yee_ms"] / max(tm["launches"], 1) / 1e3
     achieved = ab["yee_step"] / yee_avg_s / 1e9
     e = eng.energy()
+    tiling = eng.describe()
     out = {
         "metric": "Mcell-updates/s", "value": value, "unit": "Mcell-updates/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": wall / a.steps * 1e3,
@@ -154,6 +155,7 @@ def main() -> int:
         "step_breakdown_ms": {"yee_step": tm["yee_ms"] / a.steps,
                               "other_kernels_and_gaps": (tm["wall_ms"] - tm["yee_ms"]) / a.steps},
         "energy_J": e, "setup_s": {"compile_model": t_compile, "engine": t_setup},
+        "tiling": tiling,
     }
     del eng
     if not a.no_cpu:

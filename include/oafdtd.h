@@ -24,6 +24,8 @@
  *   oafd_destroy         engine object lifetime (CuPy pool frees on GC)
  *   oafd_set_field       (tests: plant a field; the reference tests write NumpyEngine.E/H)
  *   oafd_step_timed      (bench: per-kernel HIP-event timing, bench/throughput.py:34-41)
+ *   oafd_describe        (tile shape chosen by the autotuner and its timings, as JSON text)
+ *   oafd_plan            (host-only: the fused step's tile plan, for CPU coverage tests)
  */
 #ifndef OAFDTD_H
 #define OAFDTD_H
@@ -135,6 +137,14 @@ int64_t oafd_estimate_bytes(const oafd_desc* d, int dtype);
  * out[0] = summed fused-kernel ms, out[1] = wall ms of the n steps (first to last event),
  * out[2] = fused launches, out[3] = summed ms of the face/source/probe kernels. */
 int oafd_step_timed(oafd_engine* e, int64_t n, double* out);
+/* JSON text: the fused-step tile shape the autotuner chose and the timing of every candidate.
+ * OAFD_YEE_VARIANT=<n> in the environment forces candidate n (tests run each one). */
+int oafd_describe(const oafd_engine* e, char* buf, int len);
+/* Host only (no device): the fused step's tile plan for a grid. npml = slab widths
+ * (xlo xhi ylo yhi zlo zhi); kb = z chunk boundaries (info[6] + 1 values); tiles = general
+ * tiles (tx, ty, chunk, b), info[7] of them; info[0..5] = interior tx0 ty0 chunk0 nx ny nchunks. */
+int oafd_plan(int nx, int ny, int nz, int batch, const int32_t* npml, int waves, int rows_per_wave,
+              int32_t* kb, int kb_cap, int32_t* tiles, int tiles_cap, int32_t* info);
 /* e == NULL: the next oafd_create returns `code` without touching the device (CPU tests);
  * otherwise the next call on e returns `code`. */
 int oafd_debug_inject_fault(oafd_engine* e, int code);

@@ -1,0 +1,260 @@
+// Face, source, probe, DFT, energy and transfer kernels of the MI355X oa-fdtd engine.
+// Field storage (B, Z, Y, X) with x fastest (the reference layout, grid.py:3-10).
+
+#pragma once
+#include "common.hpp"
+
+namespace oafd {
+
+// ---- surface impedance (physics.cu:55-86 / reference.py:129-149): one thread per (edge, b)
+template <typename R>
+struct SibcArgs {
+    int n, np, B;
+    long long bs;
+    R inv_hs;
+    void* E[3];
+    const void* H[3];
+    const int* comp;
+    const int* flat;
+    const int* hcomp;  // (n, 2)
+    const int* hflat;  // (n, 2)
+    const R* hsign;    // (n, 2)
+    const int* mat;
+    const R* r0;
+    const R* alpha;  // (nmat, np)
+    const R* gp;
+    R* jprev;  // (B, n)
+    R* v;      // (np, B, n)
+};
+
+template <typename S, typename R>
+__global__ __launch_bounds__(256) void sibc_step(SibcArgs<R> A) {
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    const int b = blockIdx.y;
+    if (e >= A.n) return;
+    const long long bb = (long long)b * A.bs;
+    R jv = (R)0;
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        const R sg = A.hsign[2 * e + s];
+        if (sg != (R)0) jv += sg * (ldS<S, R>(A.H[A.hcomp[2 * e + s]], bb + A.hflat[2 * e + s]) * A.inv_hs);
+    }
+    const int m = A.mat[e];
+    const long long be = (long long)b * A.n + e;
+    const R dj = jv - A.jprev[be];
+    R out = A.r0[m] * jv;
+    for (int p = 0; p < A.np; ++p) {
+        const long long q = (long long)p * A.B * A.n + be;
+        const R nv = A.alpha[m * A.np + p] * dj + A.gp[p] * A.v[q];
+        A.v[q] = nv;
+        out += nv;
+    }
+    A.jprev[be] = jv;
+    stS<S, R>(A.E[A.comp[e]], bb + A.flat[e], out);
+}
+
+// ---- sources (yee.cu:96-108): one thread per unique target edge, terms in channel order.
+// When the edge is dispersive, J_p += beta_p * dE restores the Debye update's dependence on the
+// injected amount (the ADE post-update runs after injection in the reference order).
+template <typename R>
+struct InjectArgs {
+    int n, B, nch;
+    long long bs;
+    long long max_steps;
+    void* E[3];
+    const uint16_t* idx[3];
+    const int* comp;
+    const int* flat;
+    const int* start;
+    const double* amp;
+    const int* chan;
+    const double* drive;
+    const double* wave;
+    const long long* step_base;
+    int step_off;
+    int Y, X;
+    int npole, ade_first, ade_nrow, jk0, njk;
+    const R* beta;
+    R* J;
+};
+
+template <typename S, typename R>
+__global__ __launch_bounds__(256) void inject_step(InjectArgs<R> A) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= A.n) return;
+    const long long step = *A.step_base + A.step_off;
+    const int c = A.comp[t];
+    const int f = A.flat[t];
+    const int m = A.idx[c][f];
+    const bool ade = m >= A.ade_first;
+    const long long sxy = (long long)A.Y * A.X;
+    for (int b = 0; b < A.B; ++b) {
+        const long long p = (long long)b * A.bs + f;
+        R e = ldS<S, R>(A.E[c], p);
+        R added = (R)0;
+        for (int q = A.start[t]; q < A.start[t + 1]; ++q) {
+            const int ch = A.chan[q];
+            const R amt = (R)(A.drive[b * A.nch + ch] * A.wave[(long long)ch * A.max_steps + step] * A.amp[q]);
+            e += amt;
+            added += amt;
+        }
+        stS<S, R>(A.E[c], p, e);
+        if (ade) {
+            const int k = f / (int)sxy;
+            const long long jq = (long long)(k - A.jk0) * sxy + (f - (long long)k * sxy);
+            const long long jstride = (long long)A.njk * sxy;
+            const int row = m - A.ade_first;
+            for (int pp = 0; pp < A.npole; ++pp) {
+                const long long jp = ((long long)(pp * A.B + b) * 3 + c) * jstride + jq;
+                A.J[jp] += A.beta[(long long)pp * A.ade_nrow + row] * added;
+            }
+        }
+    }
+}
+
+// ---- probes (yee.cu:112-131): weighted sums in FP64, fixed-order reduction, grid (P, B)
+struct ProbeArgs {
+    int nE, nH, B;
+    long long bs;
+    double inv_hs;
+    const void* F[6];
+    const int* comp;   // (nE + nH), 0..5
+    const long long* off;  // (nE + nH + 1)
+    const int* flat;
+    const double* w;
+    double* ringE;  // (CHUNK, B, nE)
+    double* ringH;
+    int slot;
+};
+
+template <typename S>
+__global__ __launch_bounds__(256) void probe_step(ProbeArgs A) {
+    __shared__ double red[256];
+    const int p = blockIdx.x, b = blockIdx.y;
+    const int c = A.comp[p];
+    const long long bb = (long long)b * A.bs;
+    double acc = 0.0;
+    for (long long e = A.off[p] + threadIdx.x; e < A.off[p + 1]; e += 256)
+        acc += ldS<S, double>(A.F[c], bb + A.flat[e]) * A.w[e];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (p < A.nE)
+            A.ringE[((long long)A.slot * A.B + b) * A.nE + p] = red[0];
+        else
+            A.ringH[((long long)A.slot * A.B + b) * A.nH + (p - A.nE)] = red[0] * A.inv_hs;
+    }
+}
+
+// ---- running DFT with Kahan-compensated FP64 accumulation: one thread per (f, b*P + p)
+__global__ __launch_bounds__(256) void dft_accum(const double* ring, int BP, int nf, const double* omega,
+                                                 double dt, double t_off, const long long* step_base,
+                                                 int slot0, int count, double* acc, double* comp) {
+    const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (t >= (long long)nf * BP) return;
+    const int f = (int)(t / BP), q = (int)(t - (long long)f * BP);
+    const double w = omega[f];
+    const long long n0 = *step_base + slot0;
+    double sr = acc[2 * t], si = acc[2 * t + 1];
+    double cr = comp[2 * t], ci = comp[2 * t + 1];
+    for (int n = 0; n < count; ++n) {
+        const double v = ring[(long long)(slot0 + n) * BP + q] * dt;
+        const double ang = w * (((double)(n0 + n)) + t_off) * dt;
+        double s, c;
+        sincos(ang, &s, &c);
+        const double yr = v * c - cr;
+        const double tr = sr + yr;
+        cr = (tr - sr) - yr;
+        sr = tr;
+        const double yi = -v * s - ci;
+        const double ti = si + yi;
+        ci = (ti - si) - yi;
+        si = ti;
+    }
+    acc[2 * t] = sr;
+    acc[2 * t + 1] = si;
+    comp[2 * t] = cr;
+    comp[2 * t + 1] = ci;
+}
+
+__global__ void advance_step(long long* step_base, int by) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *step_base += by;
+}
+
+// ---- field energy (engine.py:440-468): one workgroup per (row j, plane k, b), fixed order
+struct EnergyArgs {
+    int X, Y, Z, B;
+    long long bs;
+    double inv_hs;
+    const void* F[6];
+    const uint16_t* idx[3];
+    const double* eps;  // relative permittivity per table row, times eps0
+    const double* nw[3];
+    const double* cw[3];
+    double mu0;
+    double* partial;
+};
+
+template <typename S>
+__global__ __launch_bounds__(256) void energy_rows(EnergyArgs A) {
+    __shared__ double red[256];
+    const int j = blockIdx.x, k = blockIdx.y, b = blockIdx.z;
+    const long long q0 = ((long long)k * A.Y + j) * A.X;
+    const long long p0 = (long long)b * A.bs + q0;
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < A.X; i += 256) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            // E_c: cell-type along c, node-type along the others
+            const double wx = c == 0 ? A.cw[0][i] : A.nw[0][i];
+            const double wy = c == 1 ? A.cw[1][j] : A.nw[1][j];
+            const double wz = c == 2 ? A.cw[2][k] : A.nw[2][k];
+            const double e = ldS<S, double>(A.F[c], p0 + i);
+            acc += 0.5 * A.eps[A.idx[c][q0 + i]] * e * e * (wx * wy * wz);
+            // H_c: node-type along c, cell-type along the others
+            const double vx = c == 0 ? A.nw[0][i] : A.cw[0][i];
+            const double vy = c == 1 ? A.nw[1][j] : A.cw[1][j];
+            const double vz = c == 2 ? A.nw[2][k] : A.cw[2][k];
+            const double h = ldS<S, double>(A.F[3 + c], p0 + i) * A.inv_hs;
+            acc += 0.5 * A.mu0 * h * h * (vx * vy * vz);
+        }
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) A.partial[((long long)b * A.Z + k) * A.Y + j] = red[0];
+}
+
+__global__ __launch_bounds__(256) void sum_partials(const double* partial, long long n, double* out) {
+    __shared__ double red[256];
+    double acc = 0.0;
+    for (long long i = threadIdx.x; i < n; i += 256) acc += partial[i];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0];
+}
+
+// ---- host transfers of one component (with the H scale undone / applied)
+template <typename S>
+__global__ __launch_bounds__(256) void to_f64(const void* src, long long n, double scale, double* dst) {
+    const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (t < n) dst[t] = ldS<S, double>(src, t) * scale;
+}
+template <typename S>
+__global__ __launch_bounds__(256) void from_f64(const double* src, long long n, double scale, void* dst) {
+    const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (t < n) stS<S, double>(dst, t, src[t] * scale);
+}
+
+}  // namespace oafd

@@ -12,13 +12,15 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
 #include <vector>
 
 #include "../../include/oafdtd.h"
-#include "kernels.hpp"
+#include "aux_kernels.hpp"
+#include "yee.hpp"
 
 namespace {
 
@@ -66,6 +68,7 @@ struct oafd_engine {
     virtual int get_field(int comp, double* out) = 0;
     virtual int set_field(int comp, const double* in) = 0;
     virtual int step_timed(int64_t n, double* out) = 0;
+    virtual std::string describe() const = 0;
     int64_t step_count = 0;
     int64_t max_steps = 0;
     int64_t peak = 0;
@@ -76,7 +79,101 @@ struct oafd_engine {
 
 namespace {
 
-template <typename S, typename R, int RPWV>
+// One tile shape of the fused step: NW wavefronts of 64 lanes, RPW rows per wavefront; the
+// interior and general instantiations share the tiling.
+template <typename R>
+struct YeeVariant {
+    int nw, rpw;
+    const void* interior;
+    const void* general;
+};
+
+template <typename S, typename R, int NW, int RPW>
+YeeVariant<R> variant() {
+    return {NW, RPW, reinterpret_cast<const void*>(&oafd::yee_fused<S, R, NW, RPW, false, true>),
+            reinterpret_cast<const void*>(&oafd::yee_fused<S, R, NW, RPW, true, true>)};
+}
+
+template <typename S, typename R>
+std::vector<YeeVariant<R>> variants() {
+    if constexpr (sizeof(R) == 8)
+        return {variant<S, R, 4, 1>(), variant<S, R, 4, 2>(), variant<S, R, 8, 1>()};
+    else
+        return {variant<S, R, 4, 2>(), variant<S, R, 4, 1>(), variant<S, R, 4, 4>(), variant<S, R, 8, 1>(),
+                variant<S, R, 8, 2>()};
+}
+
+// Work decomposition of the fused step for one tile shape.
+struct YeePlan {
+    int variant = 0, nw = 4, rpw = 2;
+    int* kb = nullptr;     // chunk boundaries
+    int4* tiles = nullptr; // general tiles
+    int n_gen = 0;
+    dim3 grid_int{0, 0, 0};
+    int tx0 = 0, ty0 = 0, zc0 = 0, nzc = 1;
+    int n_chunks = 0;
+    double ms = 0.0;  // autotuner timing of one step
+};
+
+// Host planner of the fused step's work decomposition (no device needed). Chunks in z are
+// aligned with the CPML slabs: [1, a) and [e, nz+2) always run the general kernel, [a, e) is cut
+// into interior chunks; tiles whose H cells (with halo) and E cells avoid every slab and wall run
+// the interior kernel, all others are listed for the general kernel.
+struct HostPlan {
+    std::vector<int> kb;
+    std::vector<int4> gen;
+    int tx0 = 0, ty0 = 0, zc0 = 0, nxi = 0, nyi = 0, nzc = 0;
+};
+
+HostPlan plan_tiles(int nx, int ny, int nz, int B, const int npml[3][2], int nw, int rpw) {
+    HostPlan P;
+    const int tye = nw * rpw - 1;
+    const int ntx = (nx + 1 + oafd::TXE - 1) / oafd::TXE;
+    const int nty = (ny + 1 + tye - 1) / tye;
+    const int a = npml[2][0] + 2, e = nz - npml[2][1] + 1;
+    int tx_lo = std::max((npml[0][0] + 1 + oafd::TXE - 1) / oafd::TXE, 1);
+    int tx_hi = (nx - npml[0][1] - oafd::TXE) >= 0 ? (nx - npml[0][1] - oafd::TXE) / oafd::TXE : -1;
+    int ty_lo = std::max((npml[1][0] + 1 + tye - 1) / tye, 1);
+    int ty_hi = (ny - npml[1][1] - tye) >= 0 ? (ny - npml[1][1] - tye) / tye : -1;
+    const int nxi = std::max(0, tx_hi - tx_lo + 1), nyi = std::max(0, ty_hi - ty_lo + 1);
+    P.kb.push_back(1);
+    int zc0 = 0, nzc = 0;
+    if (e - a >= 4 && nxi > 0 && nyi > 0) {
+        P.kb.push_back(a);
+        zc0 = (int)P.kb.size() - 1;
+        const long long per = (long long)nxi * nyi * B;
+        const int nck = (int)std::min<long long>(e - a, std::max<long long>(1, (8192 + per - 1) / per));
+        const int kc = std::max((e - a + nck - 1) / nck, 8);
+        for (int k = a + kc; k < e; k += kc) P.kb.push_back(k);
+        P.kb.push_back(e);
+        nzc = (int)P.kb.size() - 1 - zc0;
+        P.kb.push_back(nz + 2);
+    } else {
+        const int kc = std::max(8, (nz + 1) / 16);
+        for (int k = 1 + kc; k < nz + 2; k += kc) P.kb.push_back(k);
+        P.kb.push_back(nz + 2);
+    }
+    const int nch = (int)P.kb.size() - 1;
+    for (int bb = 0; bb < B; ++bb)
+        for (int zc = 0; zc < nch; ++zc)
+            for (int ty = 0; ty < nty; ++ty)
+                for (int tx = 0; tx < ntx; ++tx) {
+                    const bool inner = nzc > 0 && zc >= zc0 && zc < zc0 + nzc && tx >= tx_lo && tx <= tx_hi &&
+                                       ty >= ty_lo && ty <= ty_hi;
+                    if (!inner) P.gen.push_back(make_int4(tx, ty, zc, bb));
+                }
+    if (nzc > 0) {
+        P.tx0 = tx_lo;
+        P.ty0 = ty_lo;
+        P.zc0 = zc0;
+        P.nxi = nxi;
+        P.nyi = nyi;
+        P.nzc = nzc;
+    }
+    return P;
+}
+
+template <typename S, typename R>
 struct Engine final : oafd_engine {
     // geometry
     int nx, ny, nz, X, Y, Z, B;
@@ -90,12 +187,16 @@ struct Engine final : oafd_engine {
     // fields [parity][comp]
     void* E[2][3] = {};
     void* H[2][3] = {};
-    oafd::StepArgs<R> base{};
+    oafd::YeeArgs<R> base{};
+    std::vector<YeeVariant<R>> vars;
+    YeePlan plan;
+    std::vector<YeePlan> tried;
+    int npml_[3][2] = {{0, 0}, {0, 0}, {0, 0}};
     oafd::SibcArgs<R> sib{};
     oafd::InjectArgs<R> inj{};
     oafd::ProbeArgs prb{};
     oafd::EnergyArgs ena{};
-    R* psiH[2] = {nullptr, nullptr};
+    R* psiH[2][3][2] = {};
     long long* step_base = nullptr;
     // probes / DFT
     int nP[2] = {0, 0};
@@ -108,7 +209,6 @@ struct Engine final : oafd_engine {
     int64_t series_steps = 0;
     double* partial = nullptr;
     double* scalar = nullptr;
-    dim3 grid_yee;
     bool have_sibc = false, have_inj = false, have_probe = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -175,40 +275,49 @@ struct Engine final : oafd_engine {
             A.ip[a] = dev_vec(cast_vec<R>(d->inv_primal[a], len[a]));
             A.id[a] = dev_vec(cast_vec<R>(d->inv_dual[a], len[a]));
         }
+        // CPML: per axis the slab widths of each side (from the E-kind correction boxes),
+        // a psi slot map per lattice kind, and one full-extent slab array per (component, term)
+        int npml[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+        for (int n = 0; n < d->n_corr; ++n) {
+            const int32_t* r = d->corr + 12 * n;
+            if (r[0] != 1) continue;  // H boxes: cell ranges along the slab axis
+            const int a = r[2], side = r[5];
+            npml[a][side] = r[7 + 2 * a] - r[6 + 2 * a];
+        }
         for (int a = 0; a < 3; ++a) {
             A.pmc_lo[a] = d->walls[2 * a] == 1;
             A.pmc_hi[a] = d->walls[2 * a + 1] == 1;
+            for (int side = 0; side < 2; ++side) npml_[a][side] = npml[a][side];
+            const int n = a == 0 ? nx : (a == 1 ? ny : nz);
+            const int ns = npml[a][0] + npml[a][1];
+            A.ns[a] = ns;
+            A.mapE[a] = nullptr;
+            A.mapH[a] = nullptr;
             for (int q = 0; q < 3; ++q) {
                 A.pe[a][q] = d->pml[a][q] ? dev_vec(cast_vec<R>(d->pml[a][q], len[a])) : nullptr;
                 A.ph[a][q] = d->pml[a][3 + q] ? dev_vec(cast_vec<R>(d->pml[a][3 + q], len[a])) : nullptr;
             }
-        }
-        // CPML correction boxes: psi_E in place, psi_H double-buffered
-        std::memset(A.corrE, -1, sizeof(A.corrE));
-        std::memset(A.corrH, -1, sizeof(A.corrH));
-        long long offE = 0, offH = 0;
-        for (int n = 0; n < d->n_corr; ++n) {
-            const int32_t* r = d->corr + 12 * n;
-            const int kind = r[0], c = r[1], a = r[2], side = r[5];
-            const int t = (a == (c + 1) % 3) ? 0 : 1;
-            oafd::CorrBox& bx = A.box[n];
-            bx.lo[0] = r[6]; bx.n[0] = r[7] - r[6];
-            bx.lo[1] = r[8]; bx.n[1] = r[9] - r[8];
-            bx.lo[2] = r[10]; bx.n[2] = r[11] - r[10];
-            bx.vol = (long long)bx.n[0] * bx.n[1] * bx.n[2];
-            if (kind == 0) {
-                bx.off = offE;
-                offE += bx.vol * B;
-                A.corrE[c][t][side] = (signed char)n;
-            } else {
-                bx.off = offH;
-                offH += bx.vol * B;
-                A.corrH[c][t][side] = (signed char)n;
+            if (ns == 0) continue;
+            std::vector<int> me(len[a], -1), mh(len[a], -1);
+            for (int I = 1; I <= npml[a][0]; ++I) me[I] = mh[I] = I - 1;  // low slab: nodes/cells 1..np
+            for (int q = 0; q < npml[a][1]; ++q) {
+                me[n - npml[a][1] + 2 + q] = npml[a][0] + q;  // high slab nodes n-np+2 .. n+1
+                mh[n - npml[a][1] + 1 + q] = npml[a][0] + q;  // high slab cells n-np+1 .. n
             }
+            A.mapE[a] = dev_vec(me);
+            A.mapH[a] = dev_vec(mh);
         }
-        A.psiE = dev<R>((size_t)offE);
-        psiH[0] = dev<R>((size_t)offH);
-        psiH[1] = dev<R>((size_t)offH);
+        for (int c = 0; c < 3; ++c)
+            for (int t = 0; t < 2; ++t) {
+                const int a = t == 0 ? (c + 1) % 3 : (c + 2) % 3;
+                A.psiE[c][t] = nullptr;
+                psiH[0][c][t] = psiH[1][c][t] = nullptr;
+                if (A.ns[a] == 0) continue;
+                const size_t vol = (size_t)B * A.ns[a] * (a == 0 ? (size_t)Y * Z : (a == 1 ? (size_t)X * Z : (size_t)X * Y));
+                A.psiE[c][t] = dev<R>(vol);
+                psiH[0][c][t] = dev<R>(vol);
+                psiH[1][c][t] = dev<R>(vol);
+            }
         // Debye ADE
         A.npole = d->n_poles;
         A.ade_first = d->n_poles > 0 ? d->ade_first : d->n_mat + 1;
@@ -216,25 +325,16 @@ struct Engine final : oafd_engine {
         A.jk0 = d->ade_k0;
         A.njk = d->ade_nk;
         if (d->n_poles > 0) {
+            std::vector<R> kp(d->n_poles), kq(d->n_poles);
             for (int p = 0; p < d->n_poles; ++p) {
-                A.kp[p] = (R)d->kp[p];
-                A.kq[p] = (R)(0.5 * (1.0 + d->kp[p]));
+                kp[p] = (R)d->kp[p];
+                kq[p] = (R)(0.5 * (1.0 + d->kp[p]));
             }
+            A.kp = dev_vec(kp);
+            A.kq = dev_vec(kq);
             A.beta = dev_vec(cast_vec<R>(d->beta, (size_t)d->n_poles * A.ade_nrow));
             A.J = dev<R>((size_t)d->n_poles * B * 3 * A.njk * Y * X);
         }
-        // z chunking: enough workgroups to fill 256 CUs several times over
-        const int ntx = (nx + 1 + oafd::TXE - 1) / oafd::TXE;
-        const int tye = NWAVE_ROWS() - 1;
-        const int nty = (ny + 1 + tye - 1) / tye;
-        const long long tiles = (long long)ntx * nty * B;
-        int nkc = (int)std::max<long long>(1, std::min<long long>(nz + 1, (4096 + tiles - 1) / tiles));
-        int kchunk = (nz + 1 + nkc - 1) / nkc;
-        kchunk = std::max(kchunk, 4);
-        nkc = (nz + 1 + kchunk - 1) / kchunk;
-        A.kchunk = kchunk;
-        A.nkc = nkc;
-        grid_yee = dim3(ntx, nty, nkc * B);
         // SIBC
         if (d->sibc_n > 0) {
             have_sibc = true;
@@ -335,23 +435,130 @@ struct Engine final : oafd_engine {
         scalar = dev<double>(1);
         ena.partial = partial;
         HIP_TRY(hipDeviceSynchronize());
+        tune();
     }
 
-    static constexpr int NWAVE_ROWS() { return oafd::NWAVE * RPWV; }
+    // ---- fused-step planning: z chunks aligned with the CPML slabs, interior box, general list
+    YeePlan make_plan(int v) {
+        YeePlan P;
+        P.variant = v;
+        P.nw = vars[v].nw;
+        P.rpw = vars[v].rpw;
+        const HostPlan H = plan_tiles(nx, ny, nz, B, npml_, P.nw, P.rpw);
+        P.n_chunks = (int)H.kb.size() - 1;
+        P.kb = dev_vec(H.kb);
+        P.n_gen = (int)H.gen.size();
+        P.tiles = dev_vec(H.gen);
+        if (H.nzc > 0) {
+            P.grid_int = dim3(H.nxi, H.nyi, H.nzc * B);
+            P.tx0 = H.tx0;
+            P.ty0 = H.ty0;
+            P.zc0 = H.zc0;
+            P.nzc = H.nzc;
+        }
+        return P;
+    }
 
-    // ---- launches
     void launch_yee(int parity) {
-        oafd::StepArgs<R> A = base;
+        oafd::YeeArgs<R> A = base;
         for (int c = 0; c < 3; ++c) {
             A.Ein[c] = E[parity][c];
             A.Hin[c] = H[parity][c];
             A.Eout[c] = E[parity ^ 1][c];
             A.Hout[c] = H[parity ^ 1][c];
+            for (int t = 0; t < 2; ++t) {
+                A.psiHin[c][t] = psiH[parity][c][t];
+                A.psiHout[c][t] = psiH[parity ^ 1][c][t];
+            }
         }
-        A.psiHin = psiH[parity];
-        A.psiHout = psiH[parity ^ 1];
-        hipLaunchKernelGGL((oafd::yee_step<S, R, RPWV>), grid_yee, dim3(oafd::TXH, oafd::NWAVE), 0, stream, A);
-        HIP_TRY(hipGetLastError());
+        A.kb = plan.kb;
+        A.tiles = plan.tiles;
+        A.tx0 = plan.tx0;
+        A.ty0 = plan.ty0;
+        A.zc0 = plan.zc0;
+        A.nzc = plan.nzc;
+        void* args[] = {&A};
+        const dim3 block(oafd::TXH, plan.nw);
+        const YeeVariant<R>& v = vars[plan.variant];
+        if (plan.grid_int.x > 0) HIP_TRY(hipLaunchKernel(v.interior, plan.grid_int, block, args, 0, stream));
+        if (plan.n_gen > 0) HIP_TRY(hipLaunchKernel(v.general, dim3(plan.n_gen), block, args, 0, stream));
+    }
+
+    // Autotuner: time every tile shape on the (still all-zero) state and keep the fastest.
+    void tune() {
+        vars = variants<S, R>();
+        const char* force = std::getenv("OAFD_YEE_VARIANT");
+        const long long cells = (long long)bs * B;
+        if (force && *force) {
+            const int v = std::min<int>(std::max(0, std::atoi(force)), (int)vars.size() - 1);
+            plan = make_plan(v);
+            return;
+        }
+        if (cells < (1LL << 22)) {
+            plan = make_plan(0);
+            return;
+        }
+        hipEvent_t e0, e1;
+        HIP_TRY(hipEventCreate(&e0));
+        HIP_TRY(hipEventCreate(&e1));
+        int best = -1;
+        for (int v = 0; v < (int)vars.size(); ++v) {
+            plan = make_plan(v);
+            for (int s = 0; s < 2; ++s) launch_yee(s & 1);
+            HIP_TRY(hipEventRecord(e0, stream));
+            for (int s = 0; s < 6; ++s) launch_yee(s & 1);
+            HIP_TRY(hipEventRecord(e1, stream));
+            HIP_TRY(hipEventSynchronize(e1));
+            float ms = 0.f;
+            HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+            plan.ms = ms / 6.0;
+            tried.push_back(plan);
+            if (best < 0 || plan.ms < tried[best].ms) best = v;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        plan = tried[best];
+        // the tuning steps ran on zero fields with no sources, but clear every state anyway
+        const size_t nfield = (size_t)B * bs * sizeof(S);
+        for (int p = 0; p < 2; ++p)
+            for (int c = 0; c < 3; ++c) {
+                HIP_TRY(hipMemsetAsync(E[p][c], 0, nfield, stream));
+                HIP_TRY(hipMemsetAsync(H[p][c], 0, nfield, stream));
+            }
+        HIP_TRY(hipStreamSynchronize(stream));
+        zero_state();
+    }
+    void zero_state() {
+        // psi and Debye state are zero unless a source has run; reallocation-free reset
+        for (int c = 0; c < 3; ++c)
+            for (int t = 0; t < 2; ++t) {
+                const int a = t == 0 ? (c + 1) % 3 : (c + 2) % 3;
+                if (!base.psiE[c][t]) continue;
+                const size_t vol = (size_t)B * base.ns[a] *
+                                   (a == 0 ? (size_t)Y * Z : (a == 1 ? (size_t)X * Z : (size_t)X * Y)) * sizeof(R);
+                HIP_TRY(hipMemset(base.psiE[c][t], 0, vol));
+                HIP_TRY(hipMemset(psiH[0][c][t], 0, vol));
+                HIP_TRY(hipMemset(psiH[1][c][t], 0, vol));
+            }
+        if (base.J) HIP_TRY(hipMemset(base.J, 0, (size_t)base.npole * B * 3 * base.njk * Y * X * sizeof(R)));
+    }
+
+    std::string describe() const override {
+        char buf[256];
+        std::string s = "{\"variant\": ";
+        std::snprintf(buf, sizeof buf, "{\"waves\": %d, \"rows_per_wave\": %d, \"tile_e\": [%d, %d], "
+                      "\"general_tiles\": %d, \"interior_grid\": [%u, %u, %u], \"chunks\": %d}",
+                      plan.nw, plan.rpw, oafd::TXE, plan.nw * plan.rpw - 1, plan.n_gen, plan.grid_int.x,
+                      plan.grid_int.y, plan.grid_int.z, plan.n_chunks);
+        s += buf;
+        s += ", \"tuning_ms\": [";
+        for (size_t q = 0; q < tried.size(); ++q) {
+            std::snprintf(buf, sizeof buf, "%s{\"waves\": %d, \"rows_per_wave\": %d, \"ms\": %.5f}", q ? ", " : "",
+                          tried[q].nw, tried[q].rpw, tried[q].ms);
+            s += buf;
+        }
+        s += "]}";
+        return s;
     }
     void launch_rest(int slot, int parity) {
         const int out = parity ^ 1;
@@ -665,25 +872,25 @@ int oafd_create(const oafd_desc* d, int dtype, int record_series, int64_t max_st
     try {
         switch (dtype) {
             case OAFD_F32: {
-                auto* x = new Engine<float, float, 4>();
+                auto* x = new Engine<float, float>();
                 e = x;
                 x->init(d, record_series != 0, max_steps, use_graph != 0);
                 break;
             }
             case OAFD_F64: {
-                auto* x = new Engine<double, double, 2>();
+                auto* x = new Engine<double, double>();
                 e = x;
                 x->init(d, record_series != 0, max_steps, use_graph != 0);
                 break;
             }
             case OAFD_F16: {
-                auto* x = new Engine<_Float16, float, 4>();
+                auto* x = new Engine<_Float16, float>();
                 e = x;
                 x->init(d, record_series != 0, max_steps, use_graph != 0);
                 break;
             }
             default: {
-                auto* x = new Engine<__bf16, float, 4>();
+                auto* x = new Engine<__bf16, float>();
                 e = x;
                 x->init(d, record_series != 0, max_steps, use_graph != 0);
                 break;
@@ -724,6 +931,13 @@ int oafd_get_field(oafd_engine* e, int comp, double* out) { ENGINE_CALL(e, retur
 int oafd_set_field(oafd_engine* e, int comp, const double* in) { ENGINE_CALL(e, return e->set_field(comp, in)); }
 int oafd_step_timed(oafd_engine* e, int64_t n, double* out) { ENGINE_CALL(e, return e->step_timed(n, out)); }
 
+int oafd_describe(const oafd_engine* e, char* buf, int len) {
+    if (!e || !buf || len <= 0) return fail(OAFD_E_INVALID, "null engine or buffer");
+    const std::string s = e->describe();
+    std::snprintf(buf, len, "%s", s.c_str());
+    return (int)s.size() < len ? OAFD_OK : fail(OAFD_E_INVALID, "describe buffer too small");
+}
+
 int64_t oafd_series_len(const oafd_engine* e) {
     if (!e) return 0;
     return static_cast<const oafd_engine*>(e)->step_count;  // series length equals steps when recorded
@@ -733,6 +947,28 @@ int64_t oafd_peak_bytes(const oafd_engine* e) { return e ? e->peak : 0; }
 int64_t oafd_estimate_bytes(const oafd_desc* d, int dtype) {
     if (check_desc(d, dtype)) return -1;
     return estimate(d, dtype);
+}
+
+int oafd_plan(int nx, int ny, int nz, int batch, const int32_t* npml, int waves, int rows_per_wave,
+              int32_t* kb, int kb_cap, int32_t* tiles, int tiles_cap, int32_t* info) {
+    if (nx < 1 || ny < 1 || nz < 1 || batch < 1 || !npml || waves < 1 || rows_per_wave < 1 || !info)
+        return fail(OAFD_E_INVALID, "bad planner arguments");
+    int np[3][2];
+    for (int a = 0; a < 3; ++a)
+        for (int sd = 0; sd < 2; ++sd) np[a][sd] = npml[2 * a + sd];
+    const HostPlan P = plan_tiles(nx, ny, nz, batch, np, waves, rows_per_wave);
+    info[0] = P.tx0; info[1] = P.ty0; info[2] = P.zc0;
+    info[3] = P.nxi; info[4] = P.nyi; info[5] = P.nzc;
+    info[6] = (int32_t)P.kb.size() - 1;
+    info[7] = (int32_t)P.gen.size();
+    if ((int)P.kb.size() > kb_cap || (int)P.gen.size() * 4 > tiles_cap)
+        return fail(OAFD_E_INVALID, "planner output buffers too small");
+    for (size_t q = 0; q < P.kb.size(); ++q) kb[q] = P.kb[q];
+    for (size_t q = 0; q < P.gen.size(); ++q) {
+        tiles[4 * q] = P.gen[q].x; tiles[4 * q + 1] = P.gen[q].y;
+        tiles[4 * q + 2] = P.gen[q].z; tiles[4 * q + 3] = P.gen[q].w;
+    }
+    return OAFD_OK;
 }
 
 int oafd_debug_inject_fault(oafd_engine* e, int code) {

@@ -1,0 +1,406 @@
+// Fused Yee step of the MI355X oa-fdtd engine (one kernel per time step).
+//
+// Storage: one array per field component, (B, Z, Y, X) with x fastest, Z = nz+2, Y = ny+2,
+// X = pitch (the reference layout, packages/oa-fdtd/src/oa_fdtd/grid.py:3-10). Fields are
+// double-buffered: step n reads (E^n, H^{n-1/2}) from one set and writes (E^{n+1}, H^{n+1/2})
+// to the other, so one pass can recompute a neighbour's H without racing its writes.
+//
+// A workgroup of NW wavefronts owns a (TXH-1) x (NW*RPW-1) tile of E cells and marches up one
+// z chunk. Per plane k it
+//   1. holds E^n(k) of the tile plus a one-cell high-side halo in LDS (one slot) and E^n(k),
+//      E^n(k+1) of its own cells in registers (E^n(k+2) is already in flight),
+//   2. computes H^{n+1/2}(k) for the tile and its low-side halo row/column (recomputed, never
+//      exchanged) and stages it in LDS,
+//   3. overwrites the LDS slot with E^n(k+1) and advances E(k) of its own cells,
+// with two barriers per plane. H(k-1) stays in registers.
+//
+// Two instantiations per storage type: the interior kernel (GEN = false) runs on tiles that
+// touch no CPML slab and no wall, with no boundary code at all; the general kernel (GEN = true)
+// runs on the remaining tiles and adds the CPML psi recursions (psi in slab arrays addressed by a
+// 1-D slot map per axis), PMC mirror ghosts and the write masks of the staggered lattice.
+// Equations: packages/oa-fdtd/src/oa_fdtd/reference.py:152-266 (order :251-266) and
+// kernels/yee.cu:30-81, cpml :139-168; the Debye post-update (physics.cu:29-51) is folded into
+// the E update, and its dependence on the injected source is restored in inject_step.
+
+#pragma once
+#include "common.hpp"
+
+namespace oafd {
+
+constexpr int TXH = 64;       // H columns per tile = one wavefront
+constexpr int TXE = TXH - 1;  // E columns per tile
+
+template <typename R>
+struct YeeArgs {
+    int nx, ny, nz, X, Y, Z, B;
+    long long bs;
+    R dtmu;    // (dt/mu0) * hscale
+    R inv_hs;  // 1 / hscale (H is stored multiplied by hscale)
+    const void* Ein[3];
+    const void* Hin[3];
+    void* Eout[3];
+    void* Hout[3];
+    const uint16_t* idx[3];
+    const R* ca;
+    const R* cb;
+    const R* ip[3];
+    const R* id[3];
+    const int* kb;       // z chunk c covers storage planes [kb[c], kb[c+1])
+    const int4* tiles;   // general kernel: (tx, ty, chunk, b) per workgroup
+    int tx0, ty0, zc0, nzc;  // interior kernel: grid offsets, chunks per batch member
+    int pmc_lo[3], pmc_hi[3];
+    const int* mapE[3];  // per axis: psi slot of an E-type (node) index, -1 outside the slabs
+    const int* mapH[3];  // per axis: psi slot of an H-type (cell) index
+    int ns[3];           // slots per axis
+    const R* pe[3][3];   // per axis eb, ec, ek (node indices)
+    const R* ph[3][3];   // per axis hb, hc, hk (cell indices)
+    R* psiE[3][2];       // [component][curl term]
+    const R* psiHin[3][2];
+    R* psiHout[3][2];
+    int npole, ade_first, ade_nrow, jk0, njk;
+    const R* kp;
+    const R* kq;  // (1 + kp) / 2
+    const R* beta;  // (npole, ade_nrow)
+    R* J;           // (npole, B, 3, njk, Y, X)
+};
+
+// psi slab arrays: axis 0 -> (B, Z, Y, ns0), axis 1 -> (B, Z, ns1, X), axis 2 -> (B, ns2, Y, X)
+template <typename R>
+__device__ __forceinline__ long long psi_at(const YeeArgs<R>& A, int a, int s, int b, int i, int j, int k) {
+    if (a == 0) return (((long long)b * A.Z + k) * A.Y + j) * A.ns[0] + s;
+    if (a == 1) return (((long long)b * A.Z + k) * A.ns[1] + s) * A.X + i;
+    return (((long long)b * A.ns[2] + s) * A.Y + j) * (long long)A.X + i;
+}
+
+template <typename S, typename R, int NW, int RPW, bool GEN, bool ADE>
+__global__ __launch_bounds__(TXH * NW) void yee_fused(YeeArgs<R> A) {
+    constexpr int TYH = NW * RPW;  // H rows per tile
+    constexpr int TYE = TYH - 1;   // E rows per tile
+    __shared__ R sE[3][TYH + 1][TXH + 1];
+    __shared__ R sH[3][TYH][TXH];
+
+    int tx, ty, zc, b;
+    if constexpr (GEN) {
+        const int4 t = A.tiles[blockIdx.x];
+        tx = t.x; ty = t.y; zc = t.z; b = t.w;
+    } else {
+        tx = A.tx0 + blockIdx.x;
+        ty = A.ty0 + blockIdx.y;
+        zc = A.zc0 + (int)(blockIdx.z % A.nzc);
+        b = (int)(blockIdx.z / A.nzc);
+    }
+    const int lane = threadIdx.x, wv = threadIdx.y;
+    const int tid = wv * TXH + lane;
+    const int i0 = 1 + tx * TXE, j0 = 1 + ty * TYE;
+    const int kbeg = A.kb[zc], kend = A.kb[zc + 1];
+    const int kstart = kbeg > 1 ? kbeg - 1 : kbeg;
+    const int X = A.X, Y = A.Y, Z = A.Z;
+    const int sxy = X * Y;
+    const long long boff = (long long)b * A.bs;
+    const S* ein[3];
+    const S* hin[3];
+    S* eout[3];
+    S* hout[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        ein[c] = static_cast<const S*>(A.Ein[c]) + boff;
+        hin[c] = static_cast<const S*>(A.Hin[c]) + boff;
+        eout[c] = static_cast<S*>(A.Eout[c]) + boff;
+        hout[c] = static_cast<S*>(A.Hout[c]) + boff;
+    }
+    const int i = i0 - 1 + lane;
+    const bool col_ok = !GEN || i < X;
+    int jr[RPW];
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) jr[r] = j0 - 1 + wv * RPW + r;
+    // per-lane / per-row metric coefficients
+    const R ipx = col_ok ? A.ip[0][i] : (R)0;
+    const R idx_ = col_ok ? A.id[0][i] : (R)0;
+    R ipy[RPW], idy[RPW];
+#pragma unroll
+    for (int r = 0; r < RPW; ++r) {
+        const bool ok = !GEN || jr[r] < Y;
+        ipy[r] = ok ? A.ip[1][jr[r]] : (R)0;
+        idy[r] = ok ? A.id[1][jr[r]] : (R)0;
+    }
+    // halo loaders: the last wave also loads row TYH, threads tid < 3(TYH+1) load column TXH
+    const bool row_halo = wv == NW - 1;
+    const int jh = j0 - 1 + TYH;
+    const bool col_halo = tid < 3 * (TYH + 1);
+    const int hc_c = tid / (TYH + 1), hc_lj = tid - (tid / (TYH + 1)) * (TYH + 1);
+    const int ih = i0 - 1 + TXH, jhc = j0 - 1 + hc_lj;
+
+    auto ld_e = [&](int c, int ii, int jj, int kk) -> R {
+        if (GEN && (ii >= X || jj >= Y || kk >= Z)) return (R)0;
+        return ldS<S, R>(ein[c], (long long)kk * sxy + (long long)jj * X + ii);
+    };
+    R e_pf[RPW][3], e_nx[RPW][3], e_own[RPW][3];
+    R row_pf[3], row_nx[3], col_pf = 0, col_nx = 0;
+    auto load_plane = [&](int kk, R (&e)[RPW][3], R (&row)[3], R& col) {
+        const bool kin = kk < Z;
+#pragma unroll
+        for (int r = 0; r < RPW; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) e[r][c] = kin ? ld_e(c, i, jr[r], kk) : (R)0;
+        if (row_halo) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) row[c] = kin ? ld_e(c, i, jh, kk) : (R)0;
+        }
+        if (col_halo) col = kin ? ld_e(hc_c, ih, jhc, kk) : (R)0;
+    };
+    auto stage = [&](const R (&e)[RPW][3], const R (&row)[3], R col) {
+#pragma unroll
+        for (int r = 0; r < RPW; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) sE[c][wv * RPW + r][lane] = e[r][c];
+        if (row_halo) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) sE[c][TYH][lane] = row[c];
+        }
+        if (col_halo) sE[hc_c][hc_lj][TXH] = col;
+    };
+
+    // prologue: E(kstart) into LDS and registers, E(kstart+1) and E(kstart+2) in flight
+    {
+        R r3[3], cl = 0;
+        load_plane(kstart, e_own, r3, cl);
+        stage(e_own, r3, cl);
+    }
+    load_plane(kstart + 1, e_nx, row_nx, col_nx);
+
+    R hprev[RPW][3], hcur[RPW][3];
+#pragma unroll
+    for (int r = 0; r < RPW; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) hprev[r][c] = (R)0;
+
+    for (int k = kstart; k < kend; ++k) {
+        const bool own_k = k >= kbeg;
+        load_plane(k + 2, e_pf, row_pf, col_pf);
+        R hold[RPW][3];
+        int mat[RPW][3];
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) {
+            const bool ok = col_ok && (!GEN || jr[r] < Y);
+            const long long q = (long long)k * sxy + (long long)jr[r] * X + i;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                hold[r][c] = ok ? ldS<S, R>(hin[c], q) : (R)0;
+                mat[r][c] = (ok && own_k) ? (int)A.idx[c][q] : 0;
+            }
+        }
+        const R ipz = A.ip[2][k], idz = A.id[2][k];
+        __syncthreads();  // LDS E slot holds E^n(k); sH is free
+        // ---- H^{n+1/2}(k):  H_c += dtmu (d1' - d0')  (pml.py H_TERMS signs)
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) {
+            const int lj = wv * RPW + r, j = jr[r];
+            R d[3][2];
+            d[0][0] = (sE[2][lj + 1][lane] - e_own[r][2]) * ipy[r];  // Hx: dEz/dy
+            d[0][1] = (e_nx[r][1] - e_own[r][1]) * ipz;                // Hx: dEy/dz
+            d[1][0] = (e_nx[r][0] - e_own[r][0]) * ipz;                // Hy: dEx/dz
+            d[1][1] = (sE[2][lj][lane + 1] - e_own[r][2]) * ipx;      // Hy: dEz/dx
+            d[2][0] = (sE[1][lj][lane + 1] - e_own[r][1]) * ipx;      // Hz: dEy/dx
+            d[2][1] = (sE[0][lj + 1][lane] - e_own[r][0]) * ipy[r];   // Hz: dEx/dy
+            // CPML psi only for cells inside the storage: the slab index uses all three
+            // coordinates, and cells past the storage edge are never stored or read by E
+            if (GEN && i < X && j < Y) {
+                const bool own = own_k && (lane >= 1 || i == 0) && (lj >= 1 || j == 0) && i <= A.nx + 1 &&
+                                 j <= A.ny + 1;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+#pragma unroll
+                    for (int t = 0; t < 2; ++t) {
+                        const int ax = t == 0 ? (c + 1) % 3 : (c + 2) % 3;
+                        if (A.mapH[ax] == nullptr) continue;
+                        const int I = ax == 0 ? i : (ax == 1 ? j : k);
+                        const int s = A.mapH[ax][I];
+                        if (s < 0) continue;
+                        const long long q = psi_at(A, ax, s, b, i, j, k);
+                        const R psi = A.ph[ax][0][I] * A.psiHin[c][t][q] + A.ph[ax][1][I] * d[c][t];
+                        const bool valid = c == 0 ? (i >= 1 && i <= A.nx + 1 && j >= 1 && j <= A.ny && k <= A.nz)
+                                         : c == 1 ? (i >= 1 && i <= A.nx && j >= 1 && j <= A.ny + 1 && k <= A.nz)
+                                                  : (i >= 1 && i <= A.nx && j >= 1 && j <= A.ny);
+                        if (own && valid) A.psiHout[c][t][q] = psi;
+                        d[c][t] = d[c][t] + A.ph[ax][2][I] * d[c][t] + psi;
+                    }
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                R h = hold[r][c] + A.dtmu * (d[c][1] - d[c][0]);
+                if constexpr (GEN) {
+                    const bool valid = c == 0 ? (i >= 1 && i <= A.nx + 1 && j >= 1 && j <= A.ny && k <= A.nz)
+                                     : c == 1 ? (i >= 1 && i <= A.nx && j >= 1 && j <= A.ny + 1 && k <= A.nz)
+                                              : (i >= 1 && i <= A.nx && j >= 1 && j <= A.ny);
+                    if (!valid) h = (R)0;
+                    if (c < 2 && k == A.nz + 1 && A.pmc_hi[2]) h = -hprev[r][c];  // z-high mirror
+                }
+                sH[c][lj][lane] = h;
+                hcur[r][c] = h;
+            }
+        }
+        if constexpr (GEN) {
+            const bool pmcx = (A.pmc_lo[0] && i0 == 1) || (A.pmc_hi[0] && i0 - 1 + TXH > A.nx + 1);
+            const bool pmcy = (A.pmc_lo[1] && j0 == 1) || (A.pmc_hi[1] && j0 - 1 + TYH > A.ny + 1);
+            if (pmcx) {  // x walls mirror Hy, Hz (reference.py:209-221 order: x, then y, then z)
+                __syncthreads();
+                const int src = (i == 0 && A.pmc_lo[0]) ? 1 : ((i == A.nx + 1 && A.pmc_hi[0]) ? -1 : 0);
+                R v[RPW][2];
+#pragma unroll
+                for (int r = 0; r < RPW; ++r) {
+                    const int lj = wv * RPW + r;
+                    v[r][0] = src ? -sH[1][lj][lane + src] : sH[1][lj][lane];
+                    v[r][1] = src ? -sH[2][lj][lane + src] : sH[2][lj][lane];
+                }
+                __syncthreads();
+#pragma unroll
+                for (int r = 0; r < RPW; ++r) {
+                    const int lj = wv * RPW + r;
+                    if (src) {
+                        sH[1][lj][lane] = v[r][0];
+                        sH[2][lj][lane] = v[r][1];
+                    }
+                    if (src && k == A.nz + 1 && A.pmc_hi[2]) {  // z-high ghost of an x ghost
+                        sH[1][lj][lane] = -hprev[r][1];
+                    }
+                }
+            }
+            if (pmcy) {  // y walls mirror Hx, Hz
+                __syncthreads();
+                R v[RPW][2];
+                int src[RPW];
+#pragma unroll
+                for (int r = 0; r < RPW; ++r) {
+                    const int lj = wv * RPW + r, j = jr[r];
+                    src[r] = (j == 0 && A.pmc_lo[1]) ? 1 : ((j == A.ny + 1 && A.pmc_hi[1]) ? -1 : 0);
+                    v[r][0] = src[r] ? -sH[0][lj + src[r]][lane] : (R)0;
+                    v[r][1] = src[r] ? -sH[2][lj + src[r]][lane] : (R)0;
+                }
+                __syncthreads();
+#pragma unroll
+                for (int r = 0; r < RPW; ++r) {
+                    const int lj = wv * RPW + r;
+                    if (src[r]) {
+                        sH[0][lj][lane] = v[r][0];
+                        sH[2][lj][lane] = v[r][1];
+                        if (k == A.nz + 1 && A.pmc_hi[2]) sH[0][lj][lane] = -hprev[r][0];
+                    }
+                }
+            }
+            if (pmcx || pmcy) {
+                __syncthreads();
+#pragma unroll
+                for (int r = 0; r < RPW; ++r)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) hcur[r][c] = sH[c][wv * RPW + r][lane];
+            }
+            // H stores (valid cells, PMC ghosts; zeros elsewhere in the domain)
+#pragma unroll
+            for (int r = 0; r < RPW; ++r) {
+                const int lj = wv * RPW + r, j = jr[r];
+                const bool own = own_k && (lane >= 1 || i == 0) && (lj >= 1 || j == 0) && i <= A.nx + 1 &&
+                                 j <= A.ny + 1;
+                if (own) {
+                    const long long q = (long long)k * sxy + (long long)j * X + i;
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) stS<S, R>(hout[c], q, hcur[r][c]);
+                }
+            }
+            if (k == 1 && own_k) {  // z-low ghost plane (tangential H mirrored, or zero)
+#pragma unroll
+                for (int r = 0; r < RPW; ++r) {
+                    const int lj = wv * RPW + r, j = jr[r];
+                    const bool own = (lane >= 1 || i == 0) && (lj >= 1 || j == 0) && i <= A.nx + 1 && j <= A.ny + 1;
+#pragma unroll
+                    for (int c = 0; c < 2; ++c) {
+                        hprev[r][c] = A.pmc_lo[2] ? -hcur[r][c] : (R)0;
+                        if (own && A.pmc_lo[2]) stS<S, R>(hout[c], (long long)j * X + i, hprev[r][c]);
+                    }
+                    hprev[r][2] = (R)0;
+                }
+            }
+        } else {
+            if (own_k) {
+#pragma unroll
+                for (int r = 0; r < RPW; ++r) {
+                    if (lane >= 1 && wv * RPW + r >= 1) {
+                        const long long q = (long long)k * sxy + (long long)jr[r] * X + i;
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) stS<S, R>(hout[c], q, hcur[r][c]);
+                    }
+                }
+            }
+        }
+        __syncthreads();  // H(k) complete in LDS; the E slot may be overwritten
+        stage(e_nx, row_nx, col_nx);
+        // ---- E^{n+1}(k) of own cells:  E_c = ca E_c + cb (d0' - d1')  (pml.py E_TERMS signs)
+        if (own_k) {
+#pragma unroll
+            for (int r = 0; r < RPW; ++r) {
+                const int lj = wv * RPW + r, j = jr[r];
+                if (lane < 1 || lj < 1) continue;
+                if (GEN && (i > A.nx + 1 || j > A.ny + 1)) continue;
+                R d[3][2];
+                d[0][0] = (hcur[r][2] - sH[2][lj - 1][lane]) * idy[r];  // Ex: dHz/dy
+                d[0][1] = (hcur[r][1] - hprev[r][1]) * idz;              // Ex: dHy/dz
+                d[1][0] = (hcur[r][0] - hprev[r][0]) * idz;              // Ey: dHx/dz
+                d[1][1] = (hcur[r][2] - sH[2][lj][lane - 1]) * idx_;     // Ey: dHz/dx
+                d[2][0] = (hcur[r][1] - sH[1][lj][lane - 1]) * idx_;     // Ez: dHy/dx
+                d[2][1] = (hcur[r][0] - sH[0][lj - 1][lane]) * idy[r];  // Ez: dHx/dy
+                const long long q = (long long)k * sxy + (long long)j * X + i;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    if constexpr (GEN) {
+#pragma unroll
+                        for (int t = 0; t < 2; ++t) {
+                            const int ax = t == 0 ? (c + 1) % 3 : (c + 2) % 3;
+                            if (A.mapE[ax] == nullptr) continue;
+                            const int I = ax == 0 ? i : (ax == 1 ? j : k);
+                            const int s = A.mapE[ax][I];
+                            if (s < 0) continue;
+                            const long long pq = psi_at(A, ax, s, b, i, j, k);
+                            const R psi = A.pe[ax][0][I] * A.psiE[c][t][pq] + A.pe[ax][1][I] * d[c][t];
+                            A.psiE[c][t][pq] = psi;
+                            d[c][t] = d[c][t] + A.pe[ax][2][I] * d[c][t] + psi;
+                        }
+                    }
+                    const int m = mat[r][c];
+                    const R eold = e_own[r][c];
+                    R en = A.ca[m] * eold + A.cb[m] * ((d[c][0] - d[c][1]) * A.inv_hs);
+                    if constexpr (ADE) {
+                        if (m >= A.ade_first) {
+                            // E -= cb sum_p (1+k_p)/2 J_p ;  J_p = k_p J_p + beta_p (E^{n+1} - E^n)
+                            const int row = m - A.ade_first;
+                            const long long jstride = (long long)A.njk * sxy;
+                            const long long jq = (long long)(k - A.jk0) * sxy + (long long)j * X + i;
+                            R jt = (R)0;
+                            for (int p = 0; p < A.npole; ++p)
+                                jt += A.kq[p] * A.J[((long long)(p * A.B + b) * 3 + c) * jstride + jq];
+                            en = en - A.cb[m] * jt;
+                            const R de = en - eold;
+                            for (int p = 0; p < A.npole; ++p) {
+                                R* jp = A.J + ((long long)(p * A.B + b) * 3 + c) * jstride + jq;
+                                *jp = A.kp[p] * (*jp) + A.beta[(long long)p * A.ade_nrow + row] * de;
+                            }
+                        }
+                    }
+                    stS<S, R>(eout[c], q, en);
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < RPW; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                hprev[r][c] = hcur[r][c];
+                e_own[r][c] = e_nx[r][c];
+                e_nx[r][c] = e_pf[r][c];
+            }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) row_nx[c] = row_pf[c];
+        col_nx = col_pf;
+    }
+}
+
+}  // namespace oafd

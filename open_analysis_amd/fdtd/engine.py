@@ -262,6 +262,14 @@ class HipEngine:
         return {"yee_ms": float(out[0]), "wall_ms": float(out[1]), "launches": float(out[2]),
                 "gap_ms": float(out[3])}
 
+    def describe(self) -> dict[str, Any]:
+        """Fused-step tile shape chosen by the autotuner, with the timing of every candidate."""
+        import json
+
+        buf = C.create_string_buffer(8192)
+        _lib.check(self._L.oafd_describe(self._h, buf, len(buf)), "oafd_describe")
+        return json.loads(buf.value.decode())
+
     def energy(self) -> float:
         v = C.c_double()
         _lib.check(self._L.oafd_energy(self._h, C.byref(v)), "oafd_energy")

@@ -116,9 +116,29 @@ def microstrip_small(k: Classes) -> Any:
                       freqs=np.linspace(1e9, 10e9, 9))
 
 
+def interior_block(k: Classes) -> Any:
+    """150x30x26 uniform 0.5 mm mesh large enough for interior tiles: 4-cell CPML on x and z,
+    PMC on y, a single-pole Debye slab (eps_inf 3.6, delta 0.4, tau 8 ps) across the middle,
+    an SIBC copper bar on it, a lumped port under the bar end, a soft source and E/H probes."""
+    d = 0.5e-3
+    g = k["Grid"](np.arange(151) * d, np.arange(31) * d, np.arange(27) * d)
+    m = k["Model"](g, k["Waveform"]("gauss_mod", 12e9, 24e9), np.array([4e9, 11e9, 19e9]),
+                   walls=("cpml", "cpml", "pmc", "pmc", "cpml", "cpml"))
+    m.cpml = k["CpmlSpec"](cells=4)
+    m.regions.append(k["Region"]((-1, -1, 9 * d), (1, 1, 15 * d), 3.6, poles=((0.4, 8e-12),)))
+    m.conductors.append(k["Conductor"]((30 * d, 12 * d, 15 * d), (120 * d, 17 * d, 16 * d),
+                                       5.8e7, 35e-6))
+    m.ports.append(k["LumpedPort"]("p1", 2, (30 * d, 14 * d, 9 * d), (30 * d, 14 * d, 15 * d), 50.0))
+    m.sources.append(k["SoftSource"]("s", 1, (75 * d, 20 * d, 12 * d), (75 * d, 20 * d, 12 * d)))
+    m.probes.append(k["Probe"]("e", "E", 2, (100 * d, 14 * d, 12 * d), (100 * d, 14 * d, 12 * d)))
+    m.probes.append(k["Probe"]("h", "H", 0, (90 * d, 14 * d, 17 * d), (91 * d, 15 * d, 17 * d), "line"))
+    return m
+
+
 CASES = {
     "gpu_cpu": (gpu_cpu_problem, 120),
     "debye_sibc_line": (debye_sibc_line, 150),
     "cpml_box": (cpml_box, 100),
     "microstrip_small": (microstrip_small, 120),
+    "interior_block": (interior_block, 120),
 }

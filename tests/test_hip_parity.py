@@ -44,7 +44,8 @@ def _fields_err(a: dict, b: dict) -> dict[str, float]:
     return out
 
 
-@pytest.mark.parametrize("name", ["gpu_cpu", "debye_sibc_line", "cpml_box", "microstrip_small"])
+@pytest.mark.parametrize("name", ["gpu_cpu", "debye_sibc_line", "cpml_box", "microstrip_small",
+                                  "interior_block"])
 @pytest.mark.parametrize("dtype", ["float32", "float64"])
 def test_hip_matches_golden(name: str, dtype: str) -> None:
     """HIP fields and probe DFTs after N steps vs the reference NumpyEngine vectors."""
@@ -154,3 +155,25 @@ def test_max_steps_is_enforced() -> None:
     e.step(10)
     with pytest.raises(ValueError):
         e.step(1)
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("name", ["interior_block", "gpu_cpu"])
+def test_every_tile_shape_matches_golden(name: str, variant: int, monkeypatch) -> None:
+    """Each fused-step tile shape the autotuner may pick, forced through OAFD_YEE_VARIANT, on a
+    case with interior tiles (interior_block) and one with PMC ghosts (gpu_cpu)."""
+    _say(f"[variant] case={name} variant={variant}")
+    monkeypatch.setenv("OAFD_YEE_VARIANT", str(variant))
+    gold = load(name)
+    model, steps = build(name)
+    c = compile_model(model)
+    eng = HipEngine(c, "float32", max_steps=steps + 10)
+    info = eng.describe()["variant"]
+    if name == "interior_block":
+        assert info["interior_grid"][0] > 0, info
+    eng.step(steps)
+    err = field_errors(eng.get_fields(), gold)
+    assert max(err.values()) < 1e-4, (info, err)
+    for k, v in eng.results().items():
+        ref = gold["dft_" + k.replace(":", "_")]
+        assert np.abs(v - ref).max() <= 1e-4 * np.abs(ref).max(), k
