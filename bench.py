@@ -90,6 +90,16 @@ def cpu_baseline(c, seconds: float) -> dict:
                       f"nproc={os.cpu_count()}, OMP_NUM_THREADS={os.environ.get('OMP_NUM_THREADS')})"}
 
 
+def copy_rate(nbytes: int, reps: int) -> dict:
+    """Plain float4 copy kernel (copy_probe) over ``nbytes``: the HBM rate a streaming kernel
+    reaches on this device, counting read + write."""
+    from open_analysis_amd.fdtd import _lib
+
+    out = np.zeros(2)
+    _lib.check(_lib.lib().oafd_copy_bandwidth(nbytes, reps, out.ctypes.data), "copy")
+    return {"bytes": nbytes, "ms": float(out[0]), "GBps": float(out[1])}
+
+
 def traffic_from_profile(workload: str, dtype: str):
     """HBM bytes per yee_step launch measured with rocprofv3 PMC counters, if committed."""
     p = ROOT / "profiles" / "traffic.json"
@@ -108,7 +118,12 @@ def main() -> int:
     ap.add_argument("--dtype", default="float32", choices=sorted(S_FIELD))
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=float, default=0.0, metavar="GIB",
+                    help="only run the plain copy kernel over GIB gibibytes and print its rate")
     a = ap.parse_args()
+    if a.calibrate_copy > 0:
+        print(json.dumps(copy_rate(int(a.calibrate_copy * 2**30), 20)), flush=True)
+        return 0
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}); the engine has no multi-GPU "
               "path in this build.", file=sys.stderr)
@@ -137,6 +152,7 @@ def main() -> int:
     achieved = ab["yee_step"] / yee_avg_s / 1e9
     e = eng.energy()
     tiling = eng.describe()
+    copy = copy_rate(2 * 2**30, 10)
     out = {
         "metric": "Mcell-updates/s", "value": value, "unit": "Mcell-updates/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": wall / a.steps * 1e3,
@@ -150,6 +166,7 @@ def main() -> int:
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic_from_profile(a.workload, a.dtype),
                      "kernel": "yee_step", "bytes_per_launch": ab["yee_step"],
+                     "copy_kernel_GBps": copy["GBps"],
                      "avg_launch_ms": yee_avg_s * 1e3},
         "rate_at_roofline": HBM_PEAK_GBS * 1e9 / (ab["yee_step"] / (n * c.batch)) / 1e6,
         "step_breakdown_ms": {"yee_step": tm["yee_ms"] / a.steps,
