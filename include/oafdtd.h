@@ -26,6 +26,8 @@
  *   oafd_step_timed      (bench: per-kernel HIP-event timing, bench/throughput.py:34-41)
  *   oafd_describe        (tile shape chosen by the autotuner and its timings, as JSON text)
  *   oafd_plan            (host-only: the fused step's tile plan, for CPU coverage tests)
+ *   oafd_copy_bandwidth  (bench/profiling: a plain copy kernel of known size, for the HBM rate
+ *                         a streaming kernel reaches and the PMC byte-counter scale)
  */
 #ifndef OAFDTD_H
 #define OAFDTD_H
@@ -145,6 +147,9 @@ int oafd_describe(const oafd_engine* e, char* buf, int len);
  * tiles (tx, ty, chunk, b), info[7] of them; info[0..5] = interior tx0 ty0 chunk0 nx ny nchunks. */
 int oafd_plan(int nx, int ny, int nz, int batch, const int32_t* npml, int waves, int rows_per_wave,
               int32_t* kb, int kb_cap, int32_t* tiles, int tiles_cap, int32_t* info);
+/* Copies `bytes` (one read + one write per byte) `reps` times with a grid-stride float4 kernel
+ * (copy_probe). out[0] = ms per copy, out[1] = GB/s counting read + write. */
+int oafd_copy_bandwidth(int64_t bytes, int reps, double* out);
 /* e == NULL: the next oafd_create returns `code` without touching the device (CPU tests);
  * otherwise the next call on e returns `code`. */
 int oafd_debug_inject_fault(oafd_engine* e, int code);

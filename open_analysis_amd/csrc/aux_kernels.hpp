@@ -257,4 +257,11 @@ __global__ __launch_bounds__(256) void from_f64(const double* src, long long n, 
     if (t < n) stS<S, double>(dst, t, src[t] * scale);
 }
 
+// ---- calibration: a plain streaming copy (reads n floats4, writes n floats4)
+__global__ __launch_bounds__(256) void copy_probe(const float4* __restrict__ src, float4* __restrict__ dst,
+                                                  long long n) {
+    const long long stride = (long long)gridDim.x * 256;
+    for (long long t = (long long)blockIdx.x * 256 + threadIdx.x; t < n; t += stride) dst[t] = src[t];
+}
+
 }  // namespace oafd

@@ -971,6 +971,40 @@ int oafd_plan(int nx, int ny, int nz, int batch, const int32_t* npml, int waves,
     return OAFD_OK;
 }
 
+int oafd_copy_bandwidth(int64_t bytes, int reps, double* out) {
+    if (bytes < 1024 || reps < 1 || !out) return fail(OAFD_E_INVALID, "bad copy arguments");
+    if (oafd_device_count() < 1) return fail(OAFD_E_NO_DEVICE, "no ROCm device visible");
+    const long long n = bytes / 16;
+    void *a = nullptr, *b = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = OAFD_OK;
+    try {
+        HIP_TRY(hipMalloc(&a, n * 16));
+        HIP_TRY(hipMalloc(&b, n * 16));
+        HIP_TRY(hipMemset(a, 0, n * 16));
+        HIP_TRY(hipEventCreate(&e0));
+        HIP_TRY(hipEventCreate(&e1));
+        const dim3 grid(256 * 32);
+        hipLaunchKernelGGL(oafd::copy_probe, grid, dim3(256), 0, 0, (const float4*)a, (float4*)b, n);
+        HIP_TRY(hipEventRecord(e0, 0));
+        for (int r = 0; r < reps; ++r)
+            hipLaunchKernelGGL(oafd::copy_probe, grid, dim3(256), 0, 0, (const float4*)a, (float4*)b, n);
+        HIP_TRY(hipEventRecord(e1, 0));
+        HIP_TRY(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        out[0] = ms / reps;                                // ms per copy
+        out[1] = 2.0 * n * 16 / (ms / reps * 1e-3) / 1e9;  // GB/s (read + write)
+    } catch (const HipFail& f) {
+        rc = fail(f.code, f.msg);
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    return rc;
+}
+
 int oafd_debug_inject_fault(oafd_engine* e, int code) {
     if (!e) {
         g_pending_create_fault = code;

@@ -73,6 +73,7 @@ def _declare(lib: C.CDLL) -> None:
         "oafd_describe": (C.c_int, [E, C.c_char_p, C.c_int]),
         "oafd_plan": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, P, C.c_int, C.c_int, P, C.c_int,
                                 P, C.c_int, P]),
+        "oafd_copy_bandwidth": (C.c_int, [C.c_int64, C.c_int, P]),
         "oafd_debug_inject_fault": (C.c_int, [E, C.c_int]),
         "oafd_last_error": (C.c_char_p, [E]),
         "oafd_destroy": (None, [E]),
@@ -87,7 +88,8 @@ EXPORTS = (
     "oafd_version", "oafd_device_count", "oafd_device_info", "oafd_create", "oafd_step",
     "oafd_energy", "oafd_results", "oafd_series_len", "oafd_series", "oafd_get_field",
     "oafd_set_field", "oafd_step_count", "oafd_peak_bytes", "oafd_estimate_bytes",
-    "oafd_step_timed", "oafd_describe", "oafd_plan", "oafd_debug_inject_fault", "oafd_last_error", "oafd_destroy",
+    "oafd_step_timed", "oafd_describe", "oafd_plan", "oafd_copy_bandwidth",
+    "oafd_debug_inject_fault", "oafd_last_error", "oafd_destroy",
 )
 
 
