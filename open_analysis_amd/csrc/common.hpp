@@ -8,15 +8,15 @@ namespace oafd {
 constexpr int MAX_POLES = 16;
 
 // Field storage type S, arithmetic type R. 16-bit storage converts through float.
-template <typename S, typename R>
-__device__ __forceinline__ R ldS(const void* base, long long i) {
+template <typename S, typename R, typename I = long long>
+__device__ __forceinline__ R ldS(const void* base, I i) {
     if constexpr (__is_same(S, __bf16) || __is_same(S, _Float16))
         return (R)(float)(static_cast<const S*>(base)[i]);
     else
         return (R)(static_cast<const S*>(base)[i]);
 }
-template <typename S, typename R>
-__device__ __forceinline__ void stS(void* base, long long i, R v) {
+template <typename S, typename R, typename I = long long>
+__device__ __forceinline__ void stS(void* base, I i, R v) {
     if constexpr (__is_same(S, __bf16) || __is_same(S, _Float16))
         static_cast<S*>(base)[i] = (S)(float)v;
     else

@@ -90,8 +90,8 @@ struct YeeVariant {
 
 template <typename S, typename R, int NW, int RPW>
 YeeVariant<R> variant() {
-    return {NW, RPW, reinterpret_cast<const void*>(&oafd::yee_fused<S, R, NW, RPW, false, true>),
-            reinterpret_cast<const void*>(&oafd::yee_fused<S, R, NW, RPW, true, true>)};
+    return {NW, RPW, reinterpret_cast<const void*>(&oafd::yee_fused<S, R, NW, RPW, false>),
+            reinterpret_cast<const void*>(&oafd::yee_fused<S, R, NW, RPW, true>)};
 }
 
 template <typename S, typename R>
@@ -156,8 +156,8 @@ HostPlan plan_tiles(int nx, int ny, int nz, int B, const int npml[3][2], int nw,
     const int nch = (int)P.kb.size() - 1;
     for (int bb = 0; bb < B; ++bb)
         for (int zc = 0; zc < nch; ++zc)
-            for (int ty = 0; ty < nty; ++ty)
-                for (int tx = 0; tx < ntx; ++tx) {
+            for (int tx = 0; tx < ntx; ++tx)
+                for (int ty = 0; ty < nty; ++ty) {
                     const bool inner = nzc > 0 && zc >= zc0 && zc < zc0 + nzc && tx >= tx_lo && tx <= tx_hi &&
                                        ty >= ty_lo && ty <= ty_hi;
                     if (!inner) P.gen.push_back(make_int4(tx, ty, zc, bb));
@@ -196,7 +196,8 @@ struct Engine final : oafd_engine {
     oafd::InjectArgs<R> inj{};
     oafd::ProbeArgs prb{};
     oafd::EnergyArgs ena{};
-    R* psiH[2][3][2] = {};
+    R* psiH[2] = {nullptr, nullptr};
+    size_t psi_count = 0;
     long long* step_base = nullptr;
     // probes / DFT
     int nP[2] = {0, 0};
@@ -287,37 +288,43 @@ struct Engine final : oafd_engine {
         for (int a = 0; a < 3; ++a) {
             A.pmc_lo[a] = d->walls[2 * a] == 1;
             A.pmc_hi[a] = d->walls[2 * a + 1] == 1;
-            for (int side = 0; side < 2; ++side) npml_[a][side] = npml[a][side];
             const int n = a == 0 ? nx : (a == 1 ? ny : nz);
             const int ns = npml[a][0] + npml[a][1];
+            for (int side = 0; side < 2; ++side) {
+                npml_[a][side] = npml[a][side];
+                A.npml[a][side] = npml[a][side];
+            }
             A.ns[a] = ns;
-            A.mapE[a] = nullptr;
-            A.mapH[a] = nullptr;
-            for (int q = 0; q < 3; ++q) {
-                A.pe[a][q] = d->pml[a][q] ? dev_vec(cast_vec<R>(d->pml[a][q], len[a])) : nullptr;
-                A.ph[a][q] = d->pml[a][3 + q] ? dev_vec(cast_vec<R>(d->pml[a][3 + q], len[a])) : nullptr;
-            }
+            A.map[a] = nullptr;
+            A.coef[a] = nullptr;
             if (ns == 0) continue;
-            std::vector<int> me(len[a], -1), mh(len[a], -1);
-            for (int I = 1; I <= npml[a][0]; ++I) me[I] = mh[I] = I - 1;  // low slab: nodes/cells 1..np
+            // slot maps: [E-type (node) | H-type (cell)]; low slab 1..np, high slab to the wall
+            std::vector<int> mp(2 * len[a], -1);
+            for (int I = 1; I <= npml[a][0]; ++I) mp[I] = mp[len[a] + I] = I - 1;
             for (int q = 0; q < npml[a][1]; ++q) {
-                me[n - npml[a][1] + 2 + q] = npml[a][0] + q;  // high slab nodes n-np+2 .. n+1
-                mh[n - npml[a][1] + 1 + q] = npml[a][0] + q;  // high slab cells n-np+1 .. n
+                mp[n - npml[a][1] + 2 + q] = npml[a][0] + q;
+                mp[len[a] + n - npml[a][1] + 1 + q] = npml[a][0] + q;
             }
-            A.mapE[a] = dev_vec(me);
-            A.mapH[a] = dev_vec(mh);
+            A.map[a] = dev_vec(mp);
+            std::vector<R> cf(6 * (size_t)len[a], (R)0);
+            for (int q = 0; q < 6; ++q)
+                if (d->pml[a][q])
+                    for (int I = 0; I < len[a]; ++I) cf[(size_t)q * len[a] + I] = (R)d->pml[a][q][I];
+            A.coef[a] = dev_vec(cf);
         }
+        // one psi slab array per (component, curl term); E in place, H double-buffered
+        long long off = 0;
         for (int c = 0; c < 3; ++c)
             for (int t = 0; t < 2; ++t) {
                 const int a = t == 0 ? (c + 1) % 3 : (c + 2) % 3;
-                A.psiE[c][t] = nullptr;
-                psiH[0][c][t] = psiH[1][c][t] = nullptr;
+                A.poff[c][t] = off;
                 if (A.ns[a] == 0) continue;
-                const size_t vol = (size_t)B * A.ns[a] * (a == 0 ? (size_t)Y * Z : (a == 1 ? (size_t)X * Z : (size_t)X * Y));
-                A.psiE[c][t] = dev<R>(vol);
-                psiH[0][c][t] = dev<R>(vol);
-                psiH[1][c][t] = dev<R>(vol);
+                off += (long long)B * A.ns[a] * (a == 0 ? (long long)Y * Z : (a == 1 ? (long long)X * Z : (long long)X * Y));
             }
+        psi_count = (size_t)off;
+        A.psiE = dev<R>(psi_count);
+        psiH[0] = dev<R>(psi_count);
+        psiH[1] = dev<R>(psi_count);
         // Debye ADE
         A.npole = d->n_poles;
         A.ade_first = d->n_poles > 0 ? d->ade_first : d->n_mat + 1;
@@ -466,22 +473,32 @@ struct Engine final : oafd_engine {
             A.Hin[c] = H[parity][c];
             A.Eout[c] = E[parity ^ 1][c];
             A.Hout[c] = H[parity ^ 1][c];
-            for (int t = 0; t < 2; ++t) {
-                A.psiHin[c][t] = psiH[parity][c][t];
-                A.psiHout[c][t] = psiH[parity ^ 1][c][t];
-            }
         }
+        A.psiHin = psiH[parity];
+        A.psiHout = psiH[parity ^ 1];
         A.kb = plan.kb;
         A.tiles = plan.tiles;
         A.tx0 = plan.tx0;
         A.ty0 = plan.ty0;
         A.zc0 = plan.zc0;
+        A.nxi = plan.grid_int.x;
+        A.nyi = plan.grid_int.y;
         A.nzc = plan.nzc;
-        void* args[] = {&A};
         const dim3 block(oafd::TXH, plan.nw);
         const YeeVariant<R>& v = vars[plan.variant];
-        if (plan.grid_int.x > 0) HIP_TRY(hipLaunchKernel(v.interior, plan.grid_int, block, args, 0, stream));
-        if (plan.n_gen > 0) HIP_TRY(hipLaunchKernel(v.general, dim3(plan.n_gen), block, args, 0, stream));
+        const int n_int = (int)(plan.grid_int.x * plan.grid_int.y * plan.grid_int.z);
+        if (n_int > 0) {
+            A.ntiles = n_int;
+            void* args[] = {&A};
+            const unsigned nb = (unsigned)(oafd::NXCD * ((n_int + oafd::NXCD - 1) / oafd::NXCD));
+            HIP_TRY(hipLaunchKernel(v.interior, dim3(nb), block, args, 0, stream));
+        }
+        if (plan.n_gen > 0) {
+            A.ntiles = plan.n_gen;
+            void* args[] = {&A};
+            const unsigned nb = (unsigned)(oafd::NXCD * ((plan.n_gen + oafd::NXCD - 1) / oafd::NXCD));
+            HIP_TRY(hipLaunchKernel(v.general, dim3(nb), block, args, 0, stream));
+        }
     }
 
     // Autotuner: time every tile shape on the (still all-zero) state and keep the fastest.
@@ -530,16 +547,11 @@ struct Engine final : oafd_engine {
     }
     void zero_state() {
         // psi and Debye state are zero unless a source has run; reallocation-free reset
-        for (int c = 0; c < 3; ++c)
-            for (int t = 0; t < 2; ++t) {
-                const int a = t == 0 ? (c + 1) % 3 : (c + 2) % 3;
-                if (!base.psiE[c][t]) continue;
-                const size_t vol = (size_t)B * base.ns[a] *
-                                   (a == 0 ? (size_t)Y * Z : (a == 1 ? (size_t)X * Z : (size_t)X * Y)) * sizeof(R);
-                HIP_TRY(hipMemset(base.psiE[c][t], 0, vol));
-                HIP_TRY(hipMemset(psiH[0][c][t], 0, vol));
-                HIP_TRY(hipMemset(psiH[1][c][t], 0, vol));
-            }
+        if (psi_count) {
+            HIP_TRY(hipMemset(base.psiE, 0, psi_count * sizeof(R)));
+            HIP_TRY(hipMemset(psiH[0], 0, psi_count * sizeof(R)));
+            HIP_TRY(hipMemset(psiH[1], 0, psi_count * sizeof(R)));
+        }
         if (base.J) HIP_TRY(hipMemset(base.J, 0, (size_t)base.npole * B * 3 * base.njk * Y * X * sizeof(R)));
     }
 
@@ -824,6 +836,8 @@ int check_desc(const oafd_desc* d, int dtype) {
     if (d->nx < 1 || d->ny < 1 || d->nz < 1 || d->batch < 1) return fail(OAFD_E_INVALID, "empty grid or batch");
     if (d->X < d->nx + 2 || d->Y != d->ny + 2 || d->Z != d->nz + 2) return fail(OAFD_E_INVALID, "storage dims do not match the grid");
     if ((int64_t)d->X * d->Y * d->Z >= (int64_t)1 << 31) return fail(OAFD_E_INVALID, "grid exceeds 2^31 cells per batch member");
+    if (d->n_poles > 0 && (int64_t)3 * d->ade_nk * d->X * d->Y >= (int64_t)1 << 31)
+        return fail(OAFD_E_INVALID, "Debye current planes exceed 2^31 values per batch member");
     if (d->n_poles < 0 || d->n_poles > OAFD_MAX_POLES) return fail(OAFD_E_INVALID, "too many Debye poles");
     if (d->n_corr < 0 || d->n_corr > OAFD_MAX_CORR) return fail(OAFD_E_INVALID, "too many CPML corrections");
     if (d->n_mat < 1 || d->n_mat > 65536) return fail(OAFD_E_INVALID, "bad material table size");

@@ -1,4 +1,4 @@
-// Fused Yee step of the MI355X oa-fdtd engine (one kernel per time step).
+// Fused Yee step of the MI355X oa-fdtd engine (one pass over the grid per time step).
 //
 // Storage: one array per field component, (B, Z, Y, X) with x fastest, Z = nz+2, Y = ny+2,
 // X = pitch (the reference layout, packages/oa-fdtd/src/oa_fdtd/grid.py:3-10). Fields are
@@ -8,16 +8,20 @@
 // A workgroup of NW wavefronts owns a (TXH-1) x (NW*RPW-1) tile of E cells and marches up one
 // z chunk. Per plane k it
 //   1. holds E^n(k) of the tile plus a one-cell high-side halo in LDS (one slot) and E^n(k),
-//      E^n(k+1) of its own cells in registers (E^n(k+2) is already in flight),
+//      E^n(k+1) of its own cells in registers; E^n(k+2), and H^{n-1/2}, the material index and
+//      the Debye current of plane k+1, are already in flight;
 //   2. computes H^{n+1/2}(k) for the tile and its low-side halo row/column (recomputed, never
-//      exchanged) and stages it in LDS,
-//   3. overwrites the LDS slot with E^n(k+1) and advances E(k) of its own cells,
+//      exchanged) and stages it in LDS;
+//   3. overwrites the LDS slot with E^n(k+1) and advances E(k) of its own cells;
 // with two barriers per plane. H(k-1) stays in registers.
 //
 // Two instantiations per storage type: the interior kernel (GEN = false) runs on tiles that
-// touch no CPML slab and no wall, with no boundary code at all; the general kernel (GEN = true)
-// runs on the remaining tiles and adds the CPML psi recursions (psi in slab arrays addressed by a
-// 1-D slot map per axis), PMC mirror ghosts and the write masks of the staggered lattice.
+// touch no CPML slab and no wall and has no boundary code; the general kernel (GEN = true) adds
+// the CPML psi recursions (psi in slab arrays addressed by a 1-D slot map per axis, with the slot
+// and coefficient lookups hoisted out of the z loop), PMC mirror ghosts and the write masks of
+// the staggered lattice. Workgroups are mapped to tiles so that consecutive tiles of the y
+// direction (which share E halo rows) run on the same XCD.
+//
 // Equations: packages/oa-fdtd/src/oa_fdtd/reference.py:152-266 (order :251-266) and
 // kernels/yee.cu:30-81, cpml :139-168; the Debye post-update (physics.cu:29-51) is folded into
 // the E update, and its dependence on the injected source is restored in inject_step.
@@ -29,6 +33,7 @@ namespace oafd {
 
 constexpr int TXH = 64;       // H columns per tile = one wavefront
 constexpr int TXE = TXH - 1;  // E columns per tile
+constexpr int NXCD = 8;       // XCDs of an MI355X: workgroup L runs on XCD L % 8
 
 template <typename R>
 struct YeeArgs {
@@ -46,20 +51,21 @@ struct YeeArgs {
     const R* ip[3];
     const R* id[3];
     const int* kb;       // z chunk c covers storage planes [kb[c], kb[c+1])
-    const int4* tiles;   // general kernel: (tx, ty, chunk, b) per workgroup
-    int tx0, ty0, zc0, nzc;  // interior kernel: grid offsets, chunks per batch member
+    const int4* tiles;   // general kernel: (tx, ty, chunk, b), ty fastest
+    int ntiles;          // tiles of this launch
+    int tx0, ty0, zc0, nxi, nyi, nzc;  // interior kernel: tile box
     int pmc_lo[3], pmc_hi[3];
-    const int* mapE[3];  // per axis: psi slot of an E-type (node) index, -1 outside the slabs
-    const int* mapH[3];  // per axis: psi slot of an H-type (cell) index
-    int ns[3];           // slots per axis
-    const R* pe[3][3];   // per axis eb, ec, ek (node indices)
-    const R* ph[3][3];   // per axis hb, hc, hk (cell indices)
-    R* psiE[3][2];       // [component][curl term]
-    const R* psiHin[3][2];
-    R* psiHout[3][2];
+    int npml[3][2];      // CPML slab widths per axis and side (0 = none)
+    const int* map[3];   // per axis: [E-type slot (len) | H-type slot (len)], -1 outside slabs
+    const R* coef[3];    // per axis: [eb | ec | ek | hb | hc | hk], each of length len
+    int ns[3];           // psi slots per axis
+    long long poff[3][2];  // psi slab offset of (component, curl term) within a kind
+    R* psiE;
+    const R* psiHin;
+    R* psiHout;
     int npole, ade_first, ade_nrow, jk0, njk;
     const R* kp;
-    const R* kq;  // (1 + kp) / 2
+    const R* kq;    // (1 + kp) / 2
     const R* beta;  // (npole, ade_nrow)
     R* J;           // (npole, B, 3, njk, Y, X)
 };
@@ -72,22 +78,57 @@ __device__ __forceinline__ long long psi_at(const YeeArgs<R>& A, int a, int s, i
     return (((long long)b * A.ns[2] + s) * A.Y + j) * (long long)A.X + i;
 }
 
-template <typename S, typename R, int NW, int RPW, bool GEN, bool ADE>
+// Slot and (b, c, k) coefficients of one axis at one lattice index.
+template <typename R>
+struct Slab {
+    int s;
+    R b, c, k;
+};
+
+template <typename R>
+__device__ __forceinline__ Slab<R> slab_at(const YeeArgs<R>& A, int a, bool hkind, int I) {
+    Slab<R> o{-1, (R)0, (R)0, (R)0};
+    const int len = a == 0 ? A.X : (a == 1 ? A.Y : A.Z);
+    if (A.map[a] == nullptr || I < 0 || I >= len) return o;
+    o.s = A.map[a][(hkind ? len : 0) + I];
+    if (o.s >= 0) {
+        const R* cf = A.coef[a] + (hkind ? 3 * len : 0);
+        o.b = cf[I];
+        o.c = cf[len + I];
+        o.k = cf[2 * len + I];
+    }
+    return o;
+}
+
+// psi <- b psi + c d ;  d <- (1 + k) d + psi   (Roden-Gedney CPML, pml.py:90-100)
+template <typename R>
+__device__ __forceinline__ void cpml_term(R& d, const Slab<R>& sl, const R* pin, R* pout, long long q, bool write) {
+    const R psi = sl.b * pin[q] + sl.c * d;
+    if (write) pout[q] = psi;
+    d = d + sl.k * d + psi;
+}
+
+template <typename S, typename R, int NW, int RPW, bool GEN>
 __global__ __launch_bounds__(TXH * NW) void yee_fused(YeeArgs<R> A) {
     constexpr int TYH = NW * RPW;  // H rows per tile
     constexpr int TYE = TYH - 1;   // E rows per tile
     __shared__ R sE[3][TYH + 1][TXH + 1];
     __shared__ R sH[3][TYH][TXH];
 
+    // ---- XCD-aware tile: XCD x runs the contiguous tile range [x*per, (x+1)*per)
+    const int per = (A.ntiles + NXCD - 1) / NXCD;
+    const int T = (int)(blockIdx.x % NXCD) * per + (int)(blockIdx.x / NXCD);
+    if (T >= A.ntiles) return;
     int tx, ty, zc, b;
     if constexpr (GEN) {
-        const int4 t = A.tiles[blockIdx.x];
+        const int4 t = A.tiles[T];
         tx = t.x; ty = t.y; zc = t.z; b = t.w;
     } else {
-        tx = A.tx0 + blockIdx.x;
-        ty = A.ty0 + blockIdx.y;
-        zc = A.zc0 + (int)(blockIdx.z % A.nzc);
-        b = (int)(blockIdx.z / A.nzc);
+        int r = T;
+        ty = A.ty0 + r % A.nyi; r /= A.nyi;
+        tx = A.tx0 + r % A.nxi; r /= A.nxi;
+        zc = A.zc0 + r % A.nzc;
+        b = r / A.nzc;
     }
     const int lane = threadIdx.x, wv = threadIdx.y;
     const int tid = wv * TXH + lane;
@@ -113,7 +154,6 @@ __global__ __launch_bounds__(TXH * NW) void yee_fused(YeeArgs<R> A) {
     int jr[RPW];
 #pragma unroll
     for (int r = 0; r < RPW; ++r) jr[r] = j0 - 1 + wv * RPW + r;
-    // per-lane / per-row metric coefficients
     const R ipx = col_ok ? A.ip[0][i] : (R)0;
     const R idx_ = col_ok ? A.id[0][i] : (R)0;
     R ipy[RPW], idy[RPW];
@@ -123,7 +163,22 @@ __global__ __launch_bounds__(TXH * NW) void yee_fused(YeeArgs<R> A) {
         ipy[r] = ok ? A.ip[1][jr[r]] : (R)0;
         idy[r] = ok ? A.id[1][jr[r]] : (R)0;
     }
-    // halo loaders: the last wave also loads row TYH, threads tid < 3(TYH+1) load column TXH
+    // ---- CPML slabs this tile touches (workgroup-uniform) and hoisted x/y lookups
+    bool slab_x = false, slab_y = false, slab_z = false;
+    Slab<R> sxE, sxH, syE[RPW], syH[RPW];
+    if constexpr (GEN) {
+        slab_x = (A.npml[0][0] > 0 && i0 - 1 <= A.npml[0][0]) || (A.npml[0][1] > 0 && i0 + TXE >= A.nx - A.npml[0][1] + 1);
+        slab_y = (A.npml[1][0] > 0 && j0 - 1 <= A.npml[1][0]) || (A.npml[1][1] > 0 && j0 + TYE >= A.ny - A.npml[1][1] + 1);
+        slab_z = (A.npml[2][0] > 0 && kstart <= A.npml[2][0]) || (A.npml[2][1] > 0 && kend >= A.nz - A.npml[2][1] + 1);
+        sxE = slab_at(A, 0, false, slab_x ? i : -1);
+        sxH = slab_at(A, 0, true, slab_x ? i : -1);
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) {
+            syE[r] = slab_at(A, 1, false, slab_y ? jr[r] : -1);
+            syH[r] = slab_at(A, 1, true, slab_y ? jr[r] : -1);
+        }
+    }
+    // ---- halo loaders: the last wave also loads row TYH, threads tid < 3(TYH+1) column TXH
     const bool row_halo = wv == NW - 1;
     const int jh = j0 - 1 + TYH;
     const bool col_halo = tid < 3 * (TYH + 1);
@@ -132,7 +187,7 @@ __global__ __launch_bounds__(TXH * NW) void yee_fused(YeeArgs<R> A) {
 
     auto ld_e = [&](int c, int ii, int jj, int kk) -> R {
         if (GEN && (ii >= X || jj >= Y || kk >= Z)) return (R)0;
-        return ldS<S, R>(ein[c], (long long)kk * sxy + (long long)jj * X + ii);
+        return ldS<S, R>(ein[c], kk * sxy + jj * X + ii);
     };
     R e_pf[RPW][3], e_nx[RPW][3], e_own[RPW][3];
     R row_pf[3], row_nx[3], col_pf = 0, col_nx = 0;
@@ -159,14 +214,38 @@ __global__ __launch_bounds__(TXH * NW) void yee_fused(YeeArgs<R> A) {
         }
         if (col_halo) sE[hc_c][hc_lj][TXH] = col;
     };
+    // H^{n-1/2}, material rows and the single-pole Debye current of one plane
+    R hold_pf[RPW][3], jj_pf[RPW][3];
+    int mat_pf[RPW][3];
+    const bool ade1 = A.npole == 1;
+    const long long jstride = (long long)A.njk * sxy;
+    const R* Jb = ade1 ? A.J + (long long)b * 3 * jstride : nullptr;  // pole 0 of member b
+    auto load_cells = [&](int kk) {
+        const bool kin = kk < kend;
+        const bool jin = ade1 && kin && kk >= A.jk0 && kk < A.jk0 + A.njk;
+#pragma unroll
+        for (int r = 0; r < RPW; ++r) {
+            const bool ok = kin && col_ok && (!GEN || jr[r] < Y);
+            const bool ecell = ok && lane >= 1 && wv * RPW + r >= 1;  // index and J feed E only
+            const int q = kk * sxy + jr[r] * X + i;
+            const int jq = (kk - A.jk0) * sxy + jr[r] * X + i;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                hold_pf[r][c] = ok ? ldS<S, R>(hin[c], q) : (R)0;
+                mat_pf[r][c] = ecell ? (int)A.idx[c][q] : 0;
+                jj_pf[r][c] = (ecell && jin) ? Jb[c * (int)jstride + jq] : (R)0;
+            }
+        }
+    };
 
-    // prologue: E(kstart) into LDS and registers, E(kstart+1) and E(kstart+2) in flight
+    // prologue: E(kstart) into LDS and registers; E(kstart+1), plane data of kstart in flight
     {
         R r3[3], cl = 0;
         load_plane(kstart, e_own, r3, cl);
         stage(e_own, r3, cl);
     }
     load_plane(kstart + 1, e_nx, row_nx, col_nx);
+    load_cells(kstart);
 
     R hprev[RPW][3], hcur[RPW][3];
 #pragma unroll
@@ -176,20 +255,24 @@ __global__ __launch_bounds__(TXH * NW) void yee_fused(YeeArgs<R> A) {
 
     for (int k = kstart; k < kend; ++k) {
         const bool own_k = k >= kbeg;
-        load_plane(k + 2, e_pf, row_pf, col_pf);
-        R hold[RPW][3];
+        R hold[RPW][3], jcur[RPW][3];
         int mat[RPW][3];
 #pragma unroll
-        for (int r = 0; r < RPW; ++r) {
-            const bool ok = col_ok && (!GEN || jr[r] < Y);
-            const long long q = (long long)k * sxy + (long long)jr[r] * X + i;
+        for (int r = 0; r < RPW; ++r)
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                hold[r][c] = ok ? ldS<S, R>(hin[c], q) : (R)0;
-                mat[r][c] = (ok && own_k) ? (int)A.idx[c][q] : 0;
+                hold[r][c] = hold_pf[r][c];
+                mat[r][c] = mat_pf[r][c];
+                jcur[r][c] = jj_pf[r][c];
             }
-        }
+        load_plane(k + 2, e_pf, row_pf, col_pf);
+        load_cells(k + 1);
         const R ipz = A.ip[2][k], idz = A.id[2][k];
+        Slab<R> szE, szH;
+        if constexpr (GEN) {
+            szE = slab_at(A, 2, false, slab_z ? k : -1);
+            szH = slab_at(A, 2, true, slab_z ? k : -1);
+        }
         __syncthreads();  // LDS E slot holds E^n(k); sH is free
         // ---- H^{n+1/2}(k):  H_c += dtmu (d1' - d0')  (pml.py H_TERMS signs)
 #pragma unroll
@@ -202,28 +285,31 @@ __global__ __launch_bounds__(TXH * NW) void yee_fused(YeeArgs<R> A) {
             d[1][1] = (sE[2][lj][lane + 1] - e_own[r][2]) * ipx;      // Hy: dEz/dx
             d[2][0] = (sE[1][lj][lane + 1] - e_own[r][1]) * ipx;      // Hz: dEy/dx
             d[2][1] = (sE[0][lj + 1][lane] - e_own[r][0]) * ipy[r];   // Hz: dEx/dy
-            // CPML psi only for cells inside the storage: the slab index uses all three
-            // coordinates, and cells past the storage edge are never stored or read by E
             if (GEN && i < X && j < Y) {
+                // CPML only for cells inside the storage (the slab index uses all coordinates)
                 const bool own = own_k && (lane >= 1 || i == 0) && (lj >= 1 || j == 0) && i <= A.nx + 1 &&
                                  j <= A.ny + 1;
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-#pragma unroll
-                    for (int t = 0; t < 2; ++t) {
-                        const int ax = t == 0 ? (c + 1) % 3 : (c + 2) % 3;
-                        if (A.mapH[ax] == nullptr) continue;
-                        const int I = ax == 0 ? i : (ax == 1 ? j : k);
-                        const int s = A.mapH[ax][I];
-                        if (s < 0) continue;
-                        const long long q = psi_at(A, ax, s, b, i, j, k);
-                        const R psi = A.ph[ax][0][I] * A.psiHin[c][t][q] + A.ph[ax][1][I] * d[c][t];
-                        const bool valid = c == 0 ? (i >= 1 && i <= A.nx + 1 && j >= 1 && j <= A.ny && k <= A.nz)
-                                         : c == 1 ? (i >= 1 && i <= A.nx && j >= 1 && j <= A.ny + 1 && k <= A.nz)
-                                                  : (i >= 1 && i <= A.nx && j >= 1 && j <= A.ny);
-                        if (own && valid) A.psiHout[c][t][q] = psi;
-                        d[c][t] = d[c][t] + A.ph[ax][2][I] * d[c][t] + psi;
-                    }
+                const bool vx = i >= 1 && i <= A.nx + 1 && j >= 1 && j <= A.ny && k <= A.nz;
+                const bool vy = i >= 1 && i <= A.nx && j >= 1 && j <= A.ny + 1 && k <= A.nz;
+                const bool vz = i >= 1 && i <= A.nx && j >= 1 && j <= A.ny;
+                // Hx: t0 along y, t1 along z; Hy: t0 along z, t1 along x; Hz: t0 along x, t1 along y
+                if (syH[r].s >= 0) {
+                    cpml_term(d[0][0], syH[r], A.psiHin + A.poff[0][0], A.psiHout + A.poff[0][0],
+                              psi_at(A, 1, syH[r].s, b, i, j, k), own && vx);
+                    cpml_term(d[2][1], syH[r], A.psiHin + A.poff[2][1], A.psiHout + A.poff[2][1],
+                              psi_at(A, 1, syH[r].s, b, i, j, k), own && vz);
+                }
+                if (szH.s >= 0) {
+                    cpml_term(d[0][1], szH, A.psiHin + A.poff[0][1], A.psiHout + A.poff[0][1],
+                              psi_at(A, 2, szH.s, b, i, j, k), own && vx);
+                    cpml_term(d[1][0], szH, A.psiHin + A.poff[1][0], A.psiHout + A.poff[1][0],
+                              psi_at(A, 2, szH.s, b, i, j, k), own && vy);
+                }
+                if (sxH.s >= 0) {
+                    cpml_term(d[1][1], sxH, A.psiHin + A.poff[1][1], A.psiHout + A.poff[1][1],
+                              psi_at(A, 0, sxH.s, b, i, j, k), own && vy);
+                    cpml_term(d[2][0], sxH, A.psiHin + A.poff[2][0], A.psiHout + A.poff[2][0],
+                              psi_at(A, 0, sxH.s, b, i, j, k), own && vz);
                 }
             }
 #pragma unroll
@@ -260,9 +346,6 @@ __global__ __launch_bounds__(TXH * NW) void yee_fused(YeeArgs<R> A) {
                     if (src) {
                         sH[1][lj][lane] = v[r][0];
                         sH[2][lj][lane] = v[r][1];
-                    }
-                    if (src && k == A.nz + 1 && A.pmc_hi[2]) {  // z-high ghost of an x ghost
-                        sH[1][lj][lane] = -hprev[r][1];
                     }
                 }
             }
@@ -302,7 +385,7 @@ __global__ __launch_bounds__(TXH * NW) void yee_fused(YeeArgs<R> A) {
                 const bool own = own_k && (lane >= 1 || i == 0) && (lj >= 1 || j == 0) && i <= A.nx + 1 &&
                                  j <= A.ny + 1;
                 if (own) {
-                    const long long q = (long long)k * sxy + (long long)j * X + i;
+                    const int q = k * sxy + j * X + i;
 #pragma unroll
                     for (int c = 0; c < 3; ++c) stS<S, R>(hout[c], q, hcur[r][c]);
                 }
@@ -325,7 +408,7 @@ __global__ __launch_bounds__(TXH * NW) void yee_fused(YeeArgs<R> A) {
 #pragma unroll
                 for (int r = 0; r < RPW; ++r) {
                     if (lane >= 1 && wv * RPW + r >= 1) {
-                        const long long q = (long long)k * sxy + (long long)jr[r] * X + i;
+                        const int q = k * sxy + jr[r] * X + i;
 #pragma unroll
                         for (int c = 0; c < 3; ++c) stS<S, R>(hout[c], q, hcur[r][c]);
                     }
@@ -348,32 +431,39 @@ __global__ __launch_bounds__(TXH * NW) void yee_fused(YeeArgs<R> A) {
                 d[1][1] = (hcur[r][2] - sH[2][lj][lane - 1]) * idx_;     // Ey: dHz/dx
                 d[2][0] = (hcur[r][1] - sH[1][lj][lane - 1]) * idx_;     // Ez: dHy/dx
                 d[2][1] = (hcur[r][0] - sH[0][lj - 1][lane]) * idy[r];  // Ez: dHx/dy
-                const long long q = (long long)k * sxy + (long long)j * X + i;
+                if constexpr (GEN) {
+                    // Ex: t0 along y, t1 along z; Ey: t0 along z, t1 along x; Ez: t0 along x, t1 along y
+                    if (syE[r].s >= 0) {
+                        const long long q = psi_at(A, 1, syE[r].s, b, i, j, k);
+                        cpml_term(d[0][0], syE[r], A.psiE + A.poff[0][0], A.psiE + A.poff[0][0], q, true);
+                        cpml_term(d[2][1], syE[r], A.psiE + A.poff[2][1], A.psiE + A.poff[2][1], q, true);
+                    }
+                    if (szE.s >= 0) {
+                        const long long q = psi_at(A, 2, szE.s, b, i, j, k);
+                        cpml_term(d[0][1], szE, A.psiE + A.poff[0][1], A.psiE + A.poff[0][1], q, true);
+                        cpml_term(d[1][0], szE, A.psiE + A.poff[1][0], A.psiE + A.poff[1][0], q, true);
+                    }
+                    if (sxE.s >= 0) {
+                        const long long q = psi_at(A, 0, sxE.s, b, i, j, k);
+                        cpml_term(d[1][1], sxE, A.psiE + A.poff[1][1], A.psiE + A.poff[1][1], q, true);
+                        cpml_term(d[2][0], sxE, A.psiE + A.poff[2][0], A.psiE + A.poff[2][0], q, true);
+                    }
+                }
+                const int q = k * sxy + j * X + i;
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    if constexpr (GEN) {
-#pragma unroll
-                        for (int t = 0; t < 2; ++t) {
-                            const int ax = t == 0 ? (c + 1) % 3 : (c + 2) % 3;
-                            if (A.mapE[ax] == nullptr) continue;
-                            const int I = ax == 0 ? i : (ax == 1 ? j : k);
-                            const int s = A.mapE[ax][I];
-                            if (s < 0) continue;
-                            const long long pq = psi_at(A, ax, s, b, i, j, k);
-                            const R psi = A.pe[ax][0][I] * A.psiE[c][t][pq] + A.pe[ax][1][I] * d[c][t];
-                            A.psiE[c][t][pq] = psi;
-                            d[c][t] = d[c][t] + A.pe[ax][2][I] * d[c][t] + psi;
-                        }
-                    }
                     const int m = mat[r][c];
                     const R eold = e_own[r][c];
                     R en = A.ca[m] * eold + A.cb[m] * ((d[c][0] - d[c][1]) * A.inv_hs);
-                    if constexpr (ADE) {
-                        if (m >= A.ade_first) {
-                            // E -= cb sum_p (1+k_p)/2 J_p ;  J_p = k_p J_p + beta_p (E^{n+1} - E^n)
-                            const int row = m - A.ade_first;
-                            const long long jstride = (long long)A.njk * sxy;
-                            const long long jq = (long long)(k - A.jk0) * sxy + (long long)j * X + i;
+                    if (m >= A.ade_first) {
+                        // E -= cb sum_p (1+k_p)/2 J_p ;  J_p = k_p J_p + beta_p (E^{n+1} - E^n)
+                        const int row = m - A.ade_first;
+                        const int jq = (k - A.jk0) * sxy + j * X + i;
+                        if (ade1) {
+                            const R jv = jcur[r][c];
+                            en = en - A.cb[m] * (A.kq[0] * jv);
+                            const_cast<R*>(Jb)[c * (int)jstride + jq] = A.kp[0] * jv + A.beta[row] * (en - eold);
+                        } else {
                             R jt = (R)0;
                             for (int p = 0; p < A.npole; ++p)
                                 jt += A.kq[p] * A.J[((long long)(p * A.B + b) * 3 + c) * jstride + jq];
