@@ -143,9 +143,11 @@ int oafd_step_timed(oafd_engine* e, int64_t n, double* out);
  * OAFD_YEE_VARIANT=<n> in the environment forces candidate n (tests run each one). */
 int oafd_describe(const oafd_engine* e, char* buf, int len);
 /* Host only (no device): the fused step's tile plan for a grid. npml = slab widths
- * (xlo xhi ylo yhi zlo zhi); kb = z chunk boundaries (info[6] + 1 values); tiles = general
- * tiles (tx, ty, chunk, b), info[7] of them; info[0..5] = interior tx0 ty0 chunk0 nx ny nchunks. */
-int oafd_plan(int nx, int ny, int nz, int batch, const int32_t* npml, int waves, int rows_per_wave,
+ * (xlo xhi ylo yhi zlo zhi); zslab = 1 when no z wall is PMC; kb = z chunk boundaries
+ * (info[6] + 1 values); tiles = general tiles (tx, ty, chunk, b), info[7] of them;
+ * info[0..5] = interior tx0 ty0 chunk0 nx ny nchunks; info[8] = 2 when the first and last chunk's
+ * interior x/y tiles run the z-slab kernel. info holds 9 values. */
+int oafd_plan(int nx, int ny, int nz, int batch, const int32_t* npml, int waves, int rows_per_wave, int zslab,
               int32_t* kb, int kb_cap, int32_t* tiles, int tiles_cap, int32_t* info);
 /* Copies `bytes` (one read + one write per byte) `reps` times with a grid-stride float4 kernel
  * (copy_probe). out[0] = ms per copy, out[1] = GB/s counting read + write. */

@@ -86,12 +86,14 @@ struct YeeVariant {
     int nw, rpw;
     const void* interior;
     const void* general;
+    const void* zslab;
 };
 
 template <typename S, typename R, int NW, int RPW>
 YeeVariant<R> variant() {
-    return {NW, RPW, reinterpret_cast<const void*>(&oafd::yee_fused<S, R, NW, RPW, false>),
-            reinterpret_cast<const void*>(&oafd::yee_fused<S, R, NW, RPW, true>)};
+    return {NW, RPW, reinterpret_cast<const void*>(&oafd::yee_fused<S, R, NW, RPW, false, false>),
+            reinterpret_cast<const void*>(&oafd::yee_fused<S, R, NW, RPW, true, false>),
+            reinterpret_cast<const void*>(&oafd::yee_fused<S, R, NW, RPW, false, true>)};
 }
 
 template <typename S, typename R>
@@ -100,7 +102,7 @@ std::vector<YeeVariant<R>> variants() {
         return {variant<S, R, 4, 1>(), variant<S, R, 4, 2>(), variant<S, R, 8, 1>()};
     else
         return {variant<S, R, 4, 2>(), variant<S, R, 4, 1>(), variant<S, R, 4, 4>(), variant<S, R, 8, 1>(),
-                variant<S, R, 8, 2>()};
+                variant<S, R, 8, 2>(), variant<S, R, 12, 1>(), variant<S, R, 16, 1>()};
 }
 
 // Work decomposition of the fused step for one tile shape.
@@ -109,8 +111,11 @@ struct YeePlan {
     int* kb = nullptr;     // chunk boundaries
     int4* tiles = nullptr; // general tiles
     int n_gen = 0;
-    dim3 grid_int{0, 0, 0};
-    int tx0 = 0, ty0 = 0, zc0 = 0, nzc = 1;
+    int tx0 = 0, ty0 = 0, nxi = 0, nyi = 0;
+    int* zc_int = nullptr;  // interior chunks
+    int nzc = 0;
+    int* zc_zs = nullptr;   // first/last chunks run by the z-slab kernel on interior x/y tiles
+    int nzs = 0;
     int n_chunks = 0;
     double ms = 0.0;  // autotuner timing of one step
 };
@@ -123,9 +128,10 @@ struct HostPlan {
     std::vector<int> kb;
     std::vector<int4> gen;
     int tx0 = 0, ty0 = 0, zc0 = 0, nxi = 0, nyi = 0, nzc = 0;
+    std::vector<int> zs;  // chunks whose interior x/y tiles run the z-slab kernel
 };
 
-HostPlan plan_tiles(int nx, int ny, int nz, int B, const int npml[3][2], int nw, int rpw) {
+HostPlan plan_tiles(int nx, int ny, int nz, int B, const int npml[3][2], int nw, int rpw, bool zslab_ok) {
     HostPlan P;
     const int tye = nw * rpw - 1;
     const int ntx = (nx + 1 + oafd::TXE - 1) / oafd::TXE;
@@ -154,13 +160,15 @@ HostPlan plan_tiles(int nx, int ny, int nz, int B, const int npml[3][2], int nw,
         P.kb.push_back(nz + 2);
     }
     const int nch = (int)P.kb.size() - 1;
+    if (nzc > 0 && zslab_ok) P.zs = {0, nch - 1};  // the slab chunks around the interior range
     for (int bb = 0; bb < B; ++bb)
         for (int zc = 0; zc < nch; ++zc)
             for (int tx = 0; tx < ntx; ++tx)
                 for (int ty = 0; ty < nty; ++ty) {
-                    const bool inner = nzc > 0 && zc >= zc0 && zc < zc0 + nzc && tx >= tx_lo && tx <= tx_hi &&
-                                       ty >= ty_lo && ty <= ty_hi;
-                    if (!inner) P.gen.push_back(make_int4(tx, ty, zc, bb));
+                    const bool xy_in = tx >= tx_lo && tx <= tx_hi && ty >= ty_lo && ty <= ty_hi;
+                    const bool inner = nzc > 0 && zc >= zc0 && zc < zc0 + nzc && xy_in;
+                    const bool zs = nzc > 0 && zslab_ok && (zc == 0 || zc == nch - 1) && xy_in;
+                    if (!inner && !zs) P.gen.push_back(make_int4(tx, ty, zc, bb));
                 }
     if (nzc > 0) {
         P.tx0 = tx_lo;
@@ -451,17 +459,25 @@ struct Engine final : oafd_engine {
         P.variant = v;
         P.nw = vars[v].nw;
         P.rpw = vars[v].rpw;
-        const HostPlan H = plan_tiles(nx, ny, nz, B, npml_, P.nw, P.rpw);
+        const bool zslab_ok = !base.pmc_lo[2] && !base.pmc_hi[2];
+        const HostPlan H = plan_tiles(nx, ny, nz, B, npml_, P.nw, P.rpw, zslab_ok);
         P.n_chunks = (int)H.kb.size() - 1;
         P.kb = dev_vec(H.kb);
         P.n_gen = (int)H.gen.size();
         P.tiles = dev_vec(H.gen);
         if (H.nzc > 0) {
-            P.grid_int = dim3(H.nxi, H.nyi, H.nzc * B);
+            std::vector<int> zc(H.nzc);
+            for (int q = 0; q < H.nzc; ++q) zc[q] = H.zc0 + q;
+            P.zc_int = dev_vec(zc);
+            P.nzc = H.nzc;
             P.tx0 = H.tx0;
             P.ty0 = H.ty0;
-            P.zc0 = H.zc0;
-            P.nzc = H.nzc;
+            P.nxi = H.nxi;
+            P.nyi = H.nyi;
+            if (!H.zs.empty()) {
+                P.zc_zs = dev_vec(H.zs);
+                P.nzs = (int)H.zs.size();
+            }
         }
         return P;
     }
@@ -480,25 +496,27 @@ struct Engine final : oafd_engine {
         A.tiles = plan.tiles;
         A.tx0 = plan.tx0;
         A.ty0 = plan.ty0;
-        A.zc0 = plan.zc0;
-        A.nxi = plan.grid_int.x;
-        A.nyi = plan.grid_int.y;
-        A.nzc = plan.nzc;
+        A.nxi = plan.nxi;
+        A.nyi = plan.nyi;
         const dim3 block(oafd::TXH, plan.nw);
         const YeeVariant<R>& v = vars[plan.variant];
-        const int n_int = (int)(plan.grid_int.x * plan.grid_int.y * plan.grid_int.z);
-        if (n_int > 0) {
-            A.ntiles = n_int;
+        auto go = [&](const void* fn, int ntiles) {
+            A.ntiles = ntiles;
             void* args[] = {&A};
-            const unsigned nb = (unsigned)(oafd::NXCD * ((n_int + oafd::NXCD - 1) / oafd::NXCD));
-            HIP_TRY(hipLaunchKernel(v.interior, dim3(nb), block, args, 0, stream));
+            const unsigned nb = (unsigned)(oafd::NXCD * ((ntiles + oafd::NXCD - 1) / oafd::NXCD));
+            HIP_TRY(hipLaunchKernel(fn, dim3(nb), block, args, 0, stream));
+        };
+        if (plan.nzc > 0) {
+            A.zcl = plan.zc_int;
+            A.nzc = plan.nzc;
+            go(v.interior, plan.nxi * plan.nyi * plan.nzc * B);
         }
-        if (plan.n_gen > 0) {
-            A.ntiles = plan.n_gen;
-            void* args[] = {&A};
-            const unsigned nb = (unsigned)(oafd::NXCD * ((plan.n_gen + oafd::NXCD - 1) / oafd::NXCD));
-            HIP_TRY(hipLaunchKernel(v.general, dim3(nb), block, args, 0, stream));
+        if (plan.nzs > 0) {
+            A.zcl = plan.zc_zs;
+            A.nzc = plan.nzs;
+            go(v.zslab, plan.nxi * plan.nyi * plan.nzs * B);
         }
+        if (plan.n_gen > 0) go(v.general, plan.n_gen);
     }
 
     // Autotuner: time every tile shape on the (still all-zero) state and keep the fastest.
@@ -560,8 +578,8 @@ struct Engine final : oafd_engine {
         std::string s = "{\"variant\": ";
         std::snprintf(buf, sizeof buf, "{\"waves\": %d, \"rows_per_wave\": %d, \"tile_e\": [%d, %d], "
                       "\"general_tiles\": %d, \"interior_grid\": [%u, %u, %u], \"chunks\": %d}",
-                      plan.nw, plan.rpw, oafd::TXE, plan.nw * plan.rpw - 1, plan.n_gen, plan.grid_int.x,
-                      plan.grid_int.y, plan.grid_int.z, plan.n_chunks);
+                      plan.nw, plan.rpw, oafd::TXE, plan.nw * plan.rpw - 1, plan.n_gen, (unsigned)plan.nxi,
+                      (unsigned)plan.nyi, (unsigned)(plan.nzc * B), plan.n_chunks);
         s += buf;
         s += ", \"tuning_ms\": [";
         for (size_t q = 0; q < tried.size(); ++q) {
@@ -963,18 +981,19 @@ int64_t oafd_estimate_bytes(const oafd_desc* d, int dtype) {
     return estimate(d, dtype);
 }
 
-int oafd_plan(int nx, int ny, int nz, int batch, const int32_t* npml, int waves, int rows_per_wave,
+int oafd_plan(int nx, int ny, int nz, int batch, const int32_t* npml, int waves, int rows_per_wave, int zslab,
               int32_t* kb, int kb_cap, int32_t* tiles, int tiles_cap, int32_t* info) {
     if (nx < 1 || ny < 1 || nz < 1 || batch < 1 || !npml || waves < 1 || rows_per_wave < 1 || !info)
         return fail(OAFD_E_INVALID, "bad planner arguments");
     int np[3][2];
     for (int a = 0; a < 3; ++a)
         for (int sd = 0; sd < 2; ++sd) np[a][sd] = npml[2 * a + sd];
-    const HostPlan P = plan_tiles(nx, ny, nz, batch, np, waves, rows_per_wave);
+    const HostPlan P = plan_tiles(nx, ny, nz, batch, np, waves, rows_per_wave, zslab != 0);
     info[0] = P.tx0; info[1] = P.ty0; info[2] = P.zc0;
     info[3] = P.nxi; info[4] = P.nyi; info[5] = P.nzc;
     info[6] = (int32_t)P.kb.size() - 1;
     info[7] = (int32_t)P.gen.size();
+    info[8] = (int32_t)P.zs.size();  // z-slab kernel chunks: 0 and info[6]-1 when 2
     if ((int)P.kb.size() > kb_cap || (int)P.gen.size() * 4 > tiles_cap)
         return fail(OAFD_E_INVALID, "planner output buffers too small");
     for (size_t q = 0; q < P.kb.size(); ++q) kb[q] = P.kb[q];

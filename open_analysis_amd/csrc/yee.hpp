@@ -53,7 +53,8 @@ struct YeeArgs {
     const int* kb;       // z chunk c covers storage planes [kb[c], kb[c+1])
     const int4* tiles;   // general kernel: (tx, ty, chunk, b), ty fastest
     int ntiles;          // tiles of this launch
-    int tx0, ty0, zc0, nxi, nyi, nzc;  // interior kernel: tile box
+    int tx0, ty0, nxi, nyi, nzc;  // interior kernels: tile box in x, y and a chunk count
+    const int* zcl;      // interior kernels: the chunks of this launch
     int pmc_lo[3], pmc_hi[3];
     int npml[3][2];      // CPML slab widths per axis and side (0 = none)
     const int* map[3];   // per axis: [E-type slot (len) | H-type slot (len)], -1 outside slabs
@@ -108,8 +109,10 @@ __device__ __forceinline__ void cpml_term(R& d, const Slab<R>& sl, const R* pin,
     d = d + sl.k * d + psi;
 }
 
-template <typename S, typename R, int NW, int RPW, bool GEN>
+template <typename S, typename R, int NW, int RPW, bool GEN, bool ZS>
 __global__ __launch_bounds__(TXH * NW) void yee_fused(YeeArgs<R> A) {
+    // GEN: general tiles. ZS (interior tiles of the first and last z chunks): x/y interior, z CPML
+    // slab applied per plane, no wall ghosts (z walls are CPML or PEC).
     constexpr int TYH = NW * RPW;  // H rows per tile
     constexpr int TYE = TYH - 1;   // E rows per tile
     __shared__ R sE[3][TYH + 1][TXH + 1];
@@ -127,7 +130,7 @@ __global__ __launch_bounds__(TXH * NW) void yee_fused(YeeArgs<R> A) {
         int r = T;
         ty = A.ty0 + r % A.nyi; r /= A.nyi;
         tx = A.tx0 + r % A.nxi; r /= A.nxi;
-        zc = A.zc0 + r % A.nzc;
+        zc = A.zcl[r % A.nzc];
         b = r / A.nzc;
     }
     const int lane = threadIdx.x, wv = threadIdx.y;
@@ -269,9 +272,9 @@ __global__ __launch_bounds__(TXH * NW) void yee_fused(YeeArgs<R> A) {
         load_cells(k + 1);
         const R ipz = A.ip[2][k], idz = A.id[2][k];
         Slab<R> szE, szH;
-        if constexpr (GEN) {
-            szE = slab_at(A, 2, false, slab_z ? k : -1);
-            szH = slab_at(A, 2, true, slab_z ? k : -1);
+        if constexpr (GEN || ZS) {
+            szE = slab_at(A, 2, false, (!GEN || slab_z) ? k : -1);
+            szH = slab_at(A, 2, true, (!GEN || slab_z) ? k : -1);
         }
         __syncthreads();  // LDS E slot holds E^n(k); sH is free
         // ---- H^{n+1/2}(k):  H_c += dtmu (d1' - d0')  (pml.py H_TERMS signs)
@@ -312,9 +315,16 @@ __global__ __launch_bounds__(TXH * NW) void yee_fused(YeeArgs<R> A) {
                               psi_at(A, 0, sxH.s, b, i, j, k), own && vz);
                 }
             }
+            if (ZS && szH.s >= 0) {  // Hx: t1 along z; Hy: t0 along z (x, y are interior)
+                const bool own = own_k && lane >= 1 && lj >= 1;
+                const long long pq = psi_at(A, 2, szH.s, b, i, j, k);
+                cpml_term(d[0][1], szH, A.psiHin + A.poff[0][1], A.psiHout + A.poff[0][1], pq, own);
+                cpml_term(d[1][0], szH, A.psiHin + A.poff[1][0], A.psiHout + A.poff[1][0], pq, own);
+            }
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 R h = hold[r][c] + A.dtmu * (d[c][1] - d[c][0]);
+                if (ZS && c < 2 && k == A.nz + 1) h = (R)0;  // Hx, Hy are cell-type along z
                 if constexpr (GEN) {
                     const bool valid = c == 0 ? (i >= 1 && i <= A.nx + 1 && j >= 1 && j <= A.ny && k <= A.nz)
                                      : c == 1 ? (i >= 1 && i <= A.nx && j >= 1 && j <= A.ny + 1 && k <= A.nz)
@@ -405,12 +415,14 @@ __global__ __launch_bounds__(TXH * NW) void yee_fused(YeeArgs<R> A) {
             }
         } else {
             if (own_k) {
+                const int c_lo = (ZS && k == A.nz + 1) ? 2 : 0;  // no Hx, Hy on the top wall plane
 #pragma unroll
                 for (int r = 0; r < RPW; ++r) {
                     if (lane >= 1 && wv * RPW + r >= 1) {
                         const int q = k * sxy + jr[r] * X + i;
 #pragma unroll
-                        for (int c = 0; c < 3; ++c) stS<S, R>(hout[c], q, hcur[r][c]);
+                        for (int c = 0; c < 3; ++c)
+                            if (c >= c_lo) stS<S, R>(hout[c], q, hcur[r][c]);
                     }
                 }
             }
@@ -448,6 +460,11 @@ __global__ __launch_bounds__(TXH * NW) void yee_fused(YeeArgs<R> A) {
                         cpml_term(d[1][1], sxE, A.psiE + A.poff[1][1], A.psiE + A.poff[1][1], q, true);
                         cpml_term(d[2][0], sxE, A.psiE + A.poff[2][0], A.psiE + A.poff[2][0], q, true);
                     }
+                }
+                if (ZS && szE.s >= 0) {  // Ex: t1 along z; Ey: t0 along z
+                    const long long pq = psi_at(A, 2, szE.s, b, i, j, k);
+                    cpml_term(d[0][1], szE, A.psiE + A.poff[0][1], A.psiE + A.poff[0][1], pq, true);
+                    cpml_term(d[1][0], szE, A.psiE + A.poff[1][0], A.psiE + A.poff[1][0], pq, true);
                 }
                 const int q = k * sxy + j * X + i;
 #pragma unroll

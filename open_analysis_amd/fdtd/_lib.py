@@ -71,8 +71,8 @@ def _declare(lib: C.CDLL) -> None:
         "oafd_estimate_bytes": (C.c_int64, [C.POINTER(OafdDesc), C.c_int]),
         "oafd_step_timed": (C.c_int, [E, C.c_int64, P]),
         "oafd_describe": (C.c_int, [E, C.c_char_p, C.c_int]),
-        "oafd_plan": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, P, C.c_int, C.c_int, P, C.c_int,
-                                P, C.c_int, P]),
+        "oafd_plan": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, P, C.c_int, C.c_int, C.c_int, P,
+                                C.c_int, P, C.c_int, P]),
         "oafd_copy_bandwidth": (C.c_int, [C.c_int64, C.c_int, P]),
         "oafd_debug_inject_fault": (C.c_int, [E, C.c_int]),
         "oafd_last_error": (C.c_char_p, [E]),

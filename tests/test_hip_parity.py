@@ -157,7 +157,7 @@ def test_max_steps_is_enforced() -> None:
         e.step(1)
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6])
 @pytest.mark.parametrize("name", ["interior_block", "gpu_cpu"])
 def test_every_tile_shape_matches_golden(name: str, variant: int, monkeypatch) -> None:
     """Each fused-step tile shape the autotuner may pick, forced through OAFD_YEE_VARIANT, on a
