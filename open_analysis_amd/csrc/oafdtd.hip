@@ -117,6 +117,7 @@ struct YeePlan {
     int* zc_zs = nullptr;   // first/last chunks run by the z-slab kernel on interior x/y tiles
     int nzs = 0;
     int n_chunks = 0;
+    int nck = 0;      // requested interior chunks (0 = default)
     double ms = 0.0;  // autotuner timing of one step
 };
 
@@ -131,7 +132,9 @@ struct HostPlan {
     std::vector<int> zs;  // chunks whose interior x/y tiles run the z-slab kernel
 };
 
-HostPlan plan_tiles(int nx, int ny, int nz, int B, const int npml[3][2], int nw, int rpw, bool zslab_ok) {
+// nck_req: number of interior z chunks (0 = one chunk per 8192 / tiles-per-plane workgroups).
+HostPlan plan_tiles(int nx, int ny, int nz, int B, const int npml[3][2], int nw, int rpw, bool zslab_ok,
+                    int nck_req = 0) {
     HostPlan P;
     const int tye = nw * rpw - 1;
     const int ntx = (nx + 1 + oafd::TXE - 1) / oafd::TXE;
@@ -148,8 +151,9 @@ HostPlan plan_tiles(int nx, int ny, int nz, int B, const int npml[3][2], int nw,
         P.kb.push_back(a);
         zc0 = (int)P.kb.size() - 1;
         const long long per = (long long)nxi * nyi * B;
-        const int nck = (int)std::min<long long>(e - a, std::max<long long>(1, (8192 + per - 1) / per));
-        const int kc = std::max((e - a + nck - 1) / nck, 8);
+        const int want = nck_req > 0 ? nck_req : (int)std::max<long long>(1, (8192 + per - 1) / per);
+        const int nck = std::min(e - a, want);
+        const int kc = std::max((e - a + nck - 1) / nck, nck_req > 0 ? 1 : 8);
         for (int k = a + kc; k < e; k += kc) P.kb.push_back(k);
         P.kb.push_back(e);
         nzc = (int)P.kb.size() - 1 - zc0;
@@ -454,13 +458,14 @@ struct Engine final : oafd_engine {
     }
 
     // ---- fused-step planning: z chunks aligned with the CPML slabs, interior box, general list
-    YeePlan make_plan(int v) {
+    YeePlan make_plan(int v, int nck = 0) {
         YeePlan P;
         P.variant = v;
+        P.nck = nck;
         P.nw = vars[v].nw;
         P.rpw = vars[v].rpw;
         const bool zslab_ok = !base.pmc_lo[2] && !base.pmc_hi[2];
-        const HostPlan H = plan_tiles(nx, ny, nz, B, npml_, P.nw, P.rpw, zslab_ok);
+        const HostPlan H = plan_tiles(nx, ny, nz, B, npml_, P.nw, P.rpw, zslab_ok, nck);
         P.n_chunks = (int)H.kb.size() - 1;
         P.kb = dev_vec(H.kb);
         P.n_gen = (int)H.gen.size();
@@ -537,18 +542,22 @@ struct Engine final : oafd_engine {
         HIP_TRY(hipEventCreate(&e0));
         HIP_TRY(hipEventCreate(&e1));
         int best = -1;
+        static const int chunk_counts[] = {0, 1, 2, 3, 4, 6, 8, 12};
         for (int v = 0; v < (int)vars.size(); ++v) {
-            plan = make_plan(v);
-            for (int s = 0; s < 2; ++s) launch_yee(s & 1);
-            HIP_TRY(hipEventRecord(e0, stream));
-            for (int s = 0; s < 6; ++s) launch_yee(s & 1);
-            HIP_TRY(hipEventRecord(e1, stream));
-            HIP_TRY(hipEventSynchronize(e1));
-            float ms = 0.f;
-            HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-            plan.ms = ms / 6.0;
-            tried.push_back(plan);
-            if (best < 0 || plan.ms < tried[best].ms) best = v;
+            for (int nck : chunk_counts) {
+                plan = make_plan(v, nck);
+                if (nck > 0 && plan.nzc != nck) continue;  // not that many planes: same as a smaller count
+                for (int s = 0; s < 2; ++s) launch_yee(s & 1);
+                HIP_TRY(hipEventRecord(e0, stream));
+                for (int s = 0; s < 4; ++s) launch_yee(s & 1);
+                HIP_TRY(hipEventRecord(e1, stream));
+                HIP_TRY(hipEventSynchronize(e1));
+                float ms = 0.f;
+                HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+                plan.ms = ms / 4.0;
+                tried.push_back(plan);
+                if (best < 0 || plan.ms < tried[best].ms) best = (int)tried.size() - 1;
+            }
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
@@ -583,8 +592,8 @@ struct Engine final : oafd_engine {
         s += buf;
         s += ", \"tuning_ms\": [";
         for (size_t q = 0; q < tried.size(); ++q) {
-            std::snprintf(buf, sizeof buf, "%s{\"waves\": %d, \"rows_per_wave\": %d, \"ms\": %.5f}", q ? ", " : "",
-                          tried[q].nw, tried[q].rpw, tried[q].ms);
+            std::snprintf(buf, sizeof buf, "%s{\"waves\": %d, \"rows_per_wave\": %d, \"chunks\": %d, \"ms\": %.5f}",
+                          q ? ", " : "", tried[q].nw, tried[q].rpw, tried[q].nzc, tried[q].ms);
             s += buf;
         }
         s += "]}";

@@ -225,11 +225,12 @@ __global__ __launch_bounds__(TXH * NW) void yee_fused(YeeArgs<R> A) {
     const R* Jb = ade1 ? A.J + (long long)b * 3 * jstride : nullptr;  // pole 0 of member b
     auto load_cells = [&](int kk) {
         const bool kin = kk < kend;
+        const bool kown = kk >= kbeg;  // the halo plane kbeg-1 only needs H
         const bool jin = ade1 && kin && kk >= A.jk0 && kk < A.jk0 + A.njk;
 #pragma unroll
         for (int r = 0; r < RPW; ++r) {
             const bool ok = kin && col_ok && (!GEN || jr[r] < Y);
-            const bool ecell = ok && lane >= 1 && wv * RPW + r >= 1;  // index and J feed E only
+            const bool ecell = ok && kown && lane >= 1 && wv * RPW + r >= 1;  // index, J feed E only
             const int q = kk * sxy + jr[r] * X + i;
             const int jq = (kk - A.jk0) * sxy + jr[r] * X + i;
 #pragma unroll
@@ -268,7 +269,7 @@ __global__ __launch_bounds__(TXH * NW) void yee_fused(YeeArgs<R> A) {
                 mat[r][c] = mat_pf[r][c];
                 jcur[r][c] = jj_pf[r][c];
             }
-        load_plane(k + 2, e_pf, row_pf, col_pf);
+        if (k + 2 <= kend) load_plane(k + 2, e_pf, row_pf, col_pf);  // H(kend-1) needs E(kend)
         load_cells(k + 1);
         const R ipz = A.ip[2][k], idz = A.id[2][k];
         Slab<R> szE, szH;
