@@ -72,6 +72,7 @@ struct InjectArgs {
     const double* wave;
     const long long* step_base;
     int step_off;
+    R fscale;  // field storage scale
     int Y, X;
     int npole, ade_first, ade_nrow, jk0, njk;
     const R* beta;
@@ -94,7 +95,7 @@ __global__ __launch_bounds__(256) void inject_step(InjectArgs<R> A) {
         R added = (R)0;
         for (int q = A.start[t]; q < A.start[t + 1]; ++q) {
             const int ch = A.chan[q];
-            const R amt = (R)(A.drive[b * A.nch + ch] * A.wave[(long long)ch * A.max_steps + step] * A.amp[q]);
+            const R amt = (R)(A.drive[b * A.nch + ch] * A.wave[(long long)ch * A.max_steps + step] * A.amp[q]) * A.fscale;
             e += amt;
             added += amt;
         }
@@ -116,7 +117,8 @@ __global__ __launch_bounds__(256) void inject_step(InjectArgs<R> A) {
 struct ProbeArgs {
     int nE, nH, B;
     long long bs;
-    double inv_hs;
+    double inv_hs;  // 1 / (field scale x H scale)
+    double inv_fs;  // 1 / field scale
     const void* F[6];
     const int* comp;   // (nE + nH), 0..5
     const long long* off;  // (nE + nH + 1)
@@ -144,7 +146,7 @@ __global__ __launch_bounds__(256) void probe_step(ProbeArgs A) {
     }
     if (threadIdx.x == 0) {
         if (p < A.nE)
-            A.ringE[((long long)A.slot * A.B + b) * A.nE + p] = red[0];
+            A.ringE[((long long)A.slot * A.B + b) * A.nE + p] = red[0] * A.inv_fs;
         else
             A.ringH[((long long)A.slot * A.B + b) * A.nH + (p - A.nE)] = red[0] * A.inv_hs;
     }
@@ -189,7 +191,8 @@ __global__ void advance_step(long long* step_base, int by) {
 struct EnergyArgs {
     int X, Y, Z, B;
     long long bs;
-    double inv_hs;
+    double inv_hs;  // 1 / (field scale x H scale)
+    double inv_fs;  // 1 / field scale
     const void* F[6];
     const uint16_t* idx[3];
     const double* eps;  // relative permittivity per table row, times eps0
@@ -213,7 +216,7 @@ __global__ __launch_bounds__(256) void energy_rows(EnergyArgs A) {
             const double wx = c == 0 ? A.cw[0][i] : A.nw[0][i];
             const double wy = c == 1 ? A.cw[1][j] : A.nw[1][j];
             const double wz = c == 2 ? A.cw[2][k] : A.nw[2][k];
-            const double e = ldS<S, double>(A.F[c], p0 + i);
+            const double e = ldS<S, double>(A.F[c], p0 + i) * A.inv_fs;
             acc += 0.5 * A.eps[A.idx[c][q0 + i]] * e * e * (wx * wy * wz);
             // H_c: node-type along c, cell-type along the others
             const double vx = c == 0 ? A.nw[0][i] : A.cw[0][i];

@@ -16,6 +16,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/oafdtd.h"
@@ -191,7 +192,8 @@ struct Engine final : oafd_engine {
     int nx, ny, nz, X, Y, Z, B;
     long long bs;
     double dt;
-    double hs;  // H storage scale
+    double hs;  // H storage scale relative to E (eta0 for 16-bit storage)
+    double fs = 1.0;  // field storage scale (E stored as fs E, H as fs hs H)
     bool record_series;
     bool use_graph;
     hipStream_t stream = nullptr;
@@ -405,6 +407,33 @@ struct Engine final : oafd_engine {
             inj.jk0 = A.jk0; inj.njk = A.njk;
             inj.beta = A.beta; inj.J = A.J;
         }
+        // field scale for FP16 storage: the peak field of a source edge, estimated as its largest
+        // per-step injection times the pulse length in steps, maps to about 1024
+        if (std::is_same<S, _Float16>::value && d->inj_n > 0) {
+            double amax = 0.0;
+            for (int t = 0; t < d->inj_n; ++t) {
+                double sum = 0.0;
+                for (int q = d->inj_start[t]; q < d->inj_start[t + 1]; ++q) {
+                    double dmax = 0.0;
+                    for (int bb = 0; bb < B; ++bb)
+                        dmax = std::max(dmax, std::fabs(d->drive[bb * d->n_channels + d->inj_chan[q]]));
+                    sum += std::fabs(d->inj_amp[q]) * dmax;
+                }
+                amax = std::max(amax, sum);
+            }
+            double wmax = 0.0;
+            for (int64_t n = 0; n < (int64_t)d->n_channels * maxs; ++n) wmax = std::max(wmax, std::fabs(d->wave[n]));
+            int64_t width = 0;
+            for (int64_t n = 0; n < maxs; ++n)
+                for (int ch = 0; ch < d->n_channels; ++ch)
+                    if (std::fabs(d->wave[(int64_t)ch * maxs + n]) > 0.1 * wmax) {
+                        ++width;
+                        break;
+                    }
+            const double peak = amax * wmax * (double)std::max<int64_t>(width, 1);
+            if (peak > 0.0) fs = std::exp2(std::round(std::log2(1024.0 / peak)));
+        }
+        if (have_inj) inj.fscale = (R)fs;
         // probes
         nf = d->n_freq;
         std::vector<double> om(nf);
@@ -426,7 +455,8 @@ struct Engine final : oafd_engine {
                     }
                     off.push_back((long long)flat.size());
                 }
-            prb.nE = nP[0]; prb.nH = nP[1]; prb.B = B; prb.bs = bs; prb.inv_hs = 1.0 / hs;
+            prb.nE = nP[0]; prb.nH = nP[1]; prb.B = B; prb.bs = bs; prb.inv_hs = 1.0 / (hs * fs);
+            prb.inv_fs = 1.0 / fs;
             prb.comp = dev_vec(comp);
             prb.off = dev_vec(off);
             prb.flat = dev_vec(flat);
@@ -440,7 +470,8 @@ struct Engine final : oafd_engine {
             prb.ringH = ring[1];
         }
         // energy
-        ena.X = X; ena.Y = Y; ena.Z = Z; ena.B = B; ena.bs = bs; ena.inv_hs = 1.0 / hs;
+        ena.X = X; ena.Y = Y; ena.Z = Z; ena.B = B; ena.bs = bs; ena.inv_hs = 1.0 / (hs * fs);
+        ena.inv_fs = 1.0 / fs;
         for (int c = 0; c < 3; ++c) ena.idx[c] = idx[c];
         std::vector<double> eps(d->n_mat);
         for (int m = 0; m < d->n_mat; ++m) eps[m] = d->eps[m] * EPS0;
@@ -584,7 +615,9 @@ struct Engine final : oafd_engine {
 
     std::string describe() const override {
         char buf[256];
-        std::string s = "{\"variant\": ";
+        char fbuf[64];
+        std::snprintf(fbuf, sizeof fbuf, "{\"field_scale\": %.6g, ", fs);
+        std::string s = std::string(fbuf) + "\"variant\": ";
         std::snprintf(buf, sizeof buf, "{\"waves\": %d, \"rows_per_wave\": %d, \"tile_e\": [%d, %d], "
                       "\"general_tiles\": %d, \"interior_grid\": [%u, %u, %u], \"chunks\": %d}",
                       plan.nw, plan.rpw, oafd::TXE, plan.nw * plan.rpw - 1, plan.n_gen, (unsigned)plan.nxi,
@@ -808,7 +841,7 @@ struct Engine final : oafd_engine {
         double* tmp = nullptr;
         HIP_TRY(hipMalloc(&tmp, n * sizeof(double)));
         hipLaunchKernelGGL((oafd::to_f64<S>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, src, n,
-                           comp < 3 ? 1.0 : 1.0 / hs, tmp);
+                           comp < 3 ? 1.0 / fs : 1.0 / (hs * fs), tmp);
         hipError_t e1 = hipGetLastError();
         hipError_t e2 = hipMemcpyAsync(out, tmp, n * sizeof(double), hipMemcpyDeviceToHost, stream);
         hipError_t e3 = hipStreamSynchronize(stream);
@@ -828,7 +861,7 @@ struct Engine final : oafd_engine {
         HIP_TRY(hipMalloc(&tmp, n * sizeof(double)));
         hipError_t e1 = hipMemcpyAsync(tmp, in, n * sizeof(double), hipMemcpyHostToDevice, stream);
         hipLaunchKernelGGL((oafd::from_f64<S>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, tmp, n,
-                           comp < 3 ? 1.0 : hs, dst);
+                           comp < 3 ? fs : hs * fs, dst);
         hipError_t e2 = hipGetLastError();
         hipError_t e3 = hipStreamSynchronize(stream);
         (void)hipFree(tmp);
