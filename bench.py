@@ -95,9 +95,12 @@ def copy_rate(nbytes: int, reps: int) -> dict:
     reaches on this device, counting read + write."""
     from open_analysis_amd.fdtd import _lib
 
-    out = np.zeros(2)
+    out = np.zeros(4)
     _lib.check(_lib.lib().oafd_copy_bandwidth(nbytes, reps, out.ctypes.data), "copy")
-    return {"bytes": nbytes, "ms": float(out[0]), "GBps": float(out[1])}
+    best = 1 if out[3] > out[1] else 0
+    return {"bytes": nbytes, "ms": float(out[2 * best]), "GBps": float(out[2 * best + 1]),
+            "kernel": ("copy_probe", "copy_probe4")[best],
+            "GBps_per_kernel": {"copy_probe": float(out[1]), "copy_probe4": float(out[3])}}
 
 
 def traffic_from_profile(workload: str, dtype: str):

@@ -149,8 +149,9 @@ int oafd_describe(const oafd_engine* e, char* buf, int len);
  * interior x/y tiles run the z-slab kernel. info holds 9 values. */
 int oafd_plan(int nx, int ny, int nz, int batch, const int32_t* npml, int waves, int rows_per_wave, int zslab,
               int32_t* kb, int kb_cap, int32_t* tiles, int tiles_cap, int32_t* info);
-/* Copies `bytes` (one read + one write per byte) `reps` times with a grid-stride float4 kernel
- * (copy_probe). out[0] = ms per copy, out[1] = GB/s counting read + write. */
+/* Copies `bytes` (one read + one write per byte) `reps` times with two grid-stride float4
+ * kernels: copy_probe (one load in flight per thread) and copy_probe4 (four). out holds 4 values:
+ * out[0], out[2] = ms per copy, out[1], out[3] = GB/s counting read + write. */
 int oafd_copy_bandwidth(int64_t bytes, int reps, double* out);
 /* e == NULL: the next oafd_create returns `code` without touching the device (CPU tests);
  * otherwise the next call on e returns `code`. */

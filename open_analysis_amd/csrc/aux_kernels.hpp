@@ -267,4 +267,19 @@ __global__ __launch_bounds__(256) void copy_probe(const float4* __restrict__ src
     for (long long t = (long long)blockIdx.x * 256 + threadIdx.x; t < n; t += stride) dst[t] = src[t];
 }
 
+// Same copy with four independent 16-B loads in flight per thread before the stores.
+__global__ __launch_bounds__(256) void copy_probe4(const float4* __restrict__ src, float4* __restrict__ dst,
+                                                   long long n) {
+    const long long stride = (long long)gridDim.x * 256;
+    long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+    for (; t + 3 * stride < n; t += 4 * stride) {
+        const float4 a = src[t], b = src[t + stride], c = src[t + 2 * stride], d = src[t + 3 * stride];
+        dst[t] = a;
+        dst[t + stride] = b;
+        dst[t + 2 * stride] = c;
+        dst[t + 3 * stride] = d;
+    }
+    for (; t < n; t += stride) dst[t] = src[t];
+}
+
 }  // namespace oafd

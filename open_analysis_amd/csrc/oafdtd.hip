@@ -407,20 +407,11 @@ struct Engine final : oafd_engine {
             inj.jk0 = A.jk0; inj.njk = A.njk;
             inj.beta = A.beta; inj.J = A.J;
         }
-        // field scale for FP16 storage: the peak field of a source edge, estimated as its largest
-        // per-step injection times the pulse length in steps, maps to about 1024
+        // field scale for FP16 storage: the peak field of a source edge maps to about 1024. A
+        // source edge gains at most a = |amp| max|drive| max|s| per step; a loaded edge
+        // (ca < 1, a lumped port's sig_eq) settles at a / (1 - ca), the open-circuit field, and
+        // no edge exceeds a times the pulse length in steps. The smaller bound is taken.
         if (std::is_same<S, _Float16>::value && d->inj_n > 0) {
-            double amax = 0.0;
-            for (int t = 0; t < d->inj_n; ++t) {
-                double sum = 0.0;
-                for (int q = d->inj_start[t]; q < d->inj_start[t + 1]; ++q) {
-                    double dmax = 0.0;
-                    for (int bb = 0; bb < B; ++bb)
-                        dmax = std::max(dmax, std::fabs(d->drive[bb * d->n_channels + d->inj_chan[q]]));
-                    sum += std::fabs(d->inj_amp[q]) * dmax;
-                }
-                amax = std::max(amax, sum);
-            }
             double wmax = 0.0;
             for (int64_t n = 0; n < (int64_t)d->n_channels * maxs; ++n) wmax = std::max(wmax, std::fabs(d->wave[n]));
             int64_t width = 0;
@@ -430,7 +421,21 @@ struct Engine final : oafd_engine {
                         ++width;
                         break;
                     }
-            const double peak = amax * wmax * (double)std::max<int64_t>(width, 1);
+            double peak = 0.0;
+            for (int t = 0; t < d->inj_n; ++t) {
+                double sum = 0.0;
+                for (int q = d->inj_start[t]; q < d->inj_start[t + 1]; ++q) {
+                    double dmax = 0.0;
+                    for (int bb = 0; bb < B; ++bb)
+                        dmax = std::max(dmax, std::fabs(d->drive[bb * d->n_channels + d->inj_chan[q]]));
+                    sum += std::fabs(d->inj_amp[q]) * dmax;
+                }
+                const double a = sum * wmax;
+                double bound = a * (double)std::max<int64_t>(width, 1);
+                const double ca = d->ca[d->idx[d->inj_comp[t]][d->inj_flat[t]]];
+                if (ca < 1.0 && ca > -1.0) bound = std::min(bound, a / (1.0 - ca));
+                peak = std::max(peak, bound);
+            }
             if (peak > 0.0) fs = std::exp2(std::round(std::log2(1024.0 / peak)));
         }
         if (have_inj) inj.fscale = (R)fs;
@@ -1060,16 +1065,24 @@ int oafd_copy_bandwidth(int64_t bytes, int reps, double* out) {
         HIP_TRY(hipEventCreate(&e0));
         HIP_TRY(hipEventCreate(&e1));
         const dim3 grid(256 * 32);
-        hipLaunchKernelGGL(oafd::copy_probe, grid, dim3(256), 0, 0, (const float4*)a, (float4*)b, n);
-        HIP_TRY(hipEventRecord(e0, 0));
-        for (int r = 0; r < reps; ++r)
-            hipLaunchKernelGGL(oafd::copy_probe, grid, dim3(256), 0, 0, (const float4*)a, (float4*)b, n);
-        HIP_TRY(hipEventRecord(e1, 0));
-        HIP_TRY(hipEventSynchronize(e1));
-        float ms = 0.f;
-        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-        out[0] = ms / reps;                                // ms per copy
-        out[1] = 2.0 * n * 16 / (ms / reps * 1e-3) / 1e9;  // GB/s (read + write)
+        // variant 0: one 16-B load in flight per thread; variant 1: four (copy_probe4)
+        for (int v = 0; v < 2; ++v) {
+            auto launch = [&]() {
+                if (v == 0)
+                    hipLaunchKernelGGL(oafd::copy_probe, grid, dim3(256), 0, 0, (const float4*)a, (float4*)b, n);
+                else
+                    hipLaunchKernelGGL(oafd::copy_probe4, grid, dim3(256), 0, 0, (const float4*)a, (float4*)b, n);
+            };
+            launch();
+            HIP_TRY(hipEventRecord(e0, 0));
+            for (int r = 0; r < reps; ++r) launch();
+            HIP_TRY(hipEventRecord(e1, 0));
+            HIP_TRY(hipEventSynchronize(e1));
+            float ms = 0.f;
+            HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+            out[2 * v] = ms / reps;                                // ms per copy
+            out[2 * v + 1] = 2.0 * n * 16 / (ms / reps * 1e-3) / 1e9;  // GB/s (read + write)
+        }
     } catch (const HipFail& f) {
         rc = fail(f.code, f.msg);
     }

@@ -3,8 +3,13 @@
 Mirrors ``packages/oa-fdtd/src/oa_fdtd/tail.py:39-98``. The last ``window`` samples are fitted
 by ``y_n = sum_m c_m z_m^n`` (matrix pencil on the decimated window, amplitudes by least
 squares); the DFT of the continuation from sample ``S`` on is
-``sum_m c_m e^{(s_m - j w)(t_S - t_ref)} dt / (1 - e^{(s_m - j w) dt})``.
+``sum_m c_m e^{(s_m - j w)(t_S - t_ref)} e^{-j w t_ref} dt / (1 - e^{(s_m - j w) dt})``.
 Non-decaying and far out-of-band poles are dropped and counted.
+
+The factor ``e^{-j w t_ref}`` is required because the running DFT uses absolute time stamps
+while the fit's exponentials are referred to ``t_ref``; the reference's closed form
+(tail.py:89-98, and its module docstring) leaves it out, which rotates the continuation's phase
+by ``w t_ref``. This module keeps the factor; the tests pin it against the direct sum.
 """
 
 from __future__ import annotations
@@ -72,5 +77,5 @@ def extrapolate_dft(fit: TailFit, dt: float, t_next: float, freqs: FloatArray) -
     """``sum_{n >= S} y_n e^{-j w t_n} dt`` of the fitted continuation (tail.py:89-98)."""
     w = 2.0 * np.pi * np.asarray(freqs, dtype=np.float64)
     a = fit.s[None, :] - 1j * w[:, None]
-    num = fit.c[None, :] * np.exp(a * (t_next - fit.t_ref)) * dt
+    num = fit.c[None, :] * np.exp(a * (t_next - fit.t_ref) - 1j * w[:, None] * fit.t_ref) * dt
     return np.sum(num / (1.0 - np.exp(a * dt)), axis=1)

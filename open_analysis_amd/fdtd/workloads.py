@@ -117,3 +117,26 @@ def stripline_stack(nx: int = 800, ny: int = 400, nz: int = 200, d: float = 0.05
         m.ports.append(LumpedPort(f"{name}2", 2, (x1, ym, zg1), (x1, ym, zs), 50.0))
     m.excite = ("a1",)
     return m, eps
+
+
+def by_name(name: str, reduced: bool = False) -> Model:
+    """The named benchmark workload at its full size, or (``reduced``) the same structure on a
+    grid the float64 CPU oracle steps in about a minute."""
+    if name == "stripline_stack":
+        if reduced:
+            return stripline_stack(nx=120, ny=60, nz=48, d=0.25e-3, npml=8, strip_w=3, gap=3,
+                                   margin=8, freqs=np.linspace(1e9, 20e9, 39))[0]
+        return stripline_stack()[0]
+    if name == "dispersive_microstrip":
+        if reduced:
+            return dispersive_microstrip(nx=100, ny=50, nz=30, d=0.4e-3, npml=6, sub=3,
+                                         strip_w=3, margin=6, freqs=np.linspace(1e9, 20e9, 20))
+        return dispersive_microstrip()
+    if name == "microstrip":
+        if reduced:
+            return microstrip(nx=80, ny=40, nz=24, d=0.3e-3, npml=6, strip_w=2, strip_len=60,
+                              freqs=np.linspace(1e9, 10e9, 19))
+        return microstrip()
+    if name == "vacuum256":
+        return vacuum_cube(64 if reduced else 256)
+    raise ValueError(f"unknown workload {name!r}")

@@ -15,6 +15,7 @@ import pytest
 from open_analysis_amd.fdtd import compile_model, run, sparams
 from open_analysis_amd.fdtd import workloads as W
 from open_analysis_amd.fdtd.engine import HipEngine, hip_factory
+from open_analysis_amd.fdtd.precision import storage_errors
 from oracle import OracleEngine, oracle_factory
 
 pytestmark = pytest.mark.gpu
@@ -41,8 +42,7 @@ def test_microstrip_sparams_match_oracle() -> None:
     """Plain microstrip (the 160x80x48 workload's structure at 80x40x24, 0.3 mm): HIP FP32 S11/S21
     within 0.01 dB and 0.1 degree of the float64 oracle after the same 1500 steps."""
     _say("[config1-small] microstrip 80x40x24")
-    m = W.microstrip(nx=80, ny=40, nz=24, d=0.3e-3, npml=6, strip_w=2, strip_len=60,
-                     freqs=np.linspace(1e9, 10e9, 19))
+    m = W.by_name("microstrip", reduced=True)
     ref = run(m, engine=oracle_factory, dtype="float64", end_db=None, max_steps=1500)
     hip = run(m, engine=hip_factory, dtype="float32", end_db=None, max_steps=1500)
     s_ref, s_hip = sparams(m, ref), sparams(m, hip)
@@ -55,8 +55,7 @@ def test_dispersive_microstrip_sparams_match_oracle() -> None:
     """Debye substrate + SIBC copper microstrip (the 400x200x100 workload's structure at
     100x50x30, 0.4 mm): HIP FP32 within 0.01 dB and 0.1 degree of the float64 oracle."""
     _say("[config2-small] dispersive microstrip 100x50x30")
-    m = W.dispersive_microstrip(nx=100, ny=50, nz=30, d=0.4e-3, npml=6, sub=3, strip_w=3,
-                                margin=6, freqs=np.linspace(1e9, 20e9, 20))
+    m = W.by_name("dispersive_microstrip", reduced=True)
     ref = run(m, engine=oracle_factory, dtype="float64", end_db=None, max_steps=900)
     hip = run(m, engine=hip_factory, dtype="float32", end_db=None, max_steps=900)
     s_ref, s_hip = sparams(m, ref), sparams(m, hip)
@@ -84,28 +83,34 @@ def test_microstrip_full_grid_matches_oracle() -> None:
 
 
 @pytest.fixture(scope="module")
-def stack_runs():
-    """The six-layer stripline stack at 120x60x48 (0.25 mm) in every storage form, 2500 steps."""
-    m, _ = W.stripline_stack(nx=120, ny=60, nz=48, d=0.25e-3, npml=8, strip_w=3, gap=3,
-                             margin=8, freqs=np.linspace(1e9, 20e9, 39))
-    out = {}
-    for dt in ("float64", "float32", "float16", "bfloat16"):
-        _say(f"[stack] {dt}")
-        res = run(m, engine=hip_factory, dtype=dt, end_db=None, max_steps=2500)
-        out[dt] = sparams(m, res)
-    return out
+def stack_report():
+    """The six-layer stripline stack at 120x60x48 (0.25 mm), 2500 steps, in every storage form
+    against the FP64 run."""
+    m = W.by_name("stripline_stack", reduced=True)
+    rep = storage_errors(m, 2500, ("float32", "float16", "bfloat16"), log=sys.stdout)
+    for dt, v in rep["variants"].items():
+        for k, e in v["entries"].items():
+            _say(f"  {dt:9s} {k:10s} {e['max_db']:.2e} dB {e['max_deg']:.2e} deg "
+                 f"(|S| {e['ref_min_db']:.1f}..{e['ref_max_db']:.1f} dB)")
+    return rep
 
 
-def test_stack_float32_matches_float64(stack_runs) -> None:
-    ddb, dph = _compare(stack_runs["float32"][:, 0], stack_runs["float64"][:, 0])
-    _say(f"  float32 vs float64: {ddb:.2e} dB, {dph:.2e} deg")
-    assert ddb < 0.01 and dph < 0.1
+def test_stack_float32_matches_float64(stack_report) -> None:
+    v = stack_report["variants"]["float32"]
+    assert v["worst_db"] < 0.01 and v["worst_deg"] < 0.1
 
 
-@pytest.mark.parametrize("dtype,db_tol,deg_tol", [("float16", 0.05, 0.5), ("bfloat16", 0.5, 5.0)])
-def test_stack_half_storage_error(stack_runs, dtype: str, db_tol: float, deg_tol: float) -> None:
-    """16-bit field storage (FP32 arithmetic, FP64 DFT): S21 and S11 error against the FP64 run.
-    The bounds are the measured errors with margin; the values are printed per run."""
-    ddb, dph = _compare(stack_runs[dtype][:, 0], stack_runs["float64"][:, 0])
-    _say(f"  {dtype} vs float64: {ddb:.3e} dB, {dph:.3e} deg")
-    assert ddb < db_tol and dph < deg_tol
+# (through-entry dB, deg; every entry dB, deg): FP16 keeps the through loss within the 0.05 dB
+# target; the low-level entries and BF16's 8-bit mantissa are looser.
+BOUNDS = {"float16": (0.02, 0.3, 0.4, 5.0), "bfloat16": (0.2, 2.5, 1.5, 15.0)}
+
+
+@pytest.mark.parametrize("dtype", sorted(BOUNDS))
+def test_stack_half_storage_error(stack_report, dtype: str) -> None:
+    """16-bit field storage (FP32 arithmetic, FP64 DFT): S-parameter error against the FP64 run;
+    the through entry S(a2,a1) is the one the 0.05 dB target is stated for."""
+    v = stack_report["variants"][dtype]
+    thr = v["entries"]["S(a2,a1)"]
+    b = BOUNDS[dtype]
+    assert thr["max_db"] < b[0] and thr["max_deg"] < b[1]
+    assert v["worst_db"] < b[2] and v["worst_deg"] < b[3]

@@ -1,0 +1,202 @@
+"""The host driver around the engine (CPU only): the end criterion and ``FDTD.UNSTABLE``
+detector, the matrix-pencil tail, batching, and the material fits.
+
+The scripted-energy cases follow packages/oa-fdtd/tests/test_stability.py:51-75, the subset
+batch check test_batching.py:17-34 and the fit bounds test_fits_and_modes.py:15-34.
+"""
+
+from __future__ import annotations
+
+import copy
+from typing import Any
+
+import numpy as np
+import pytest
+from helpers import line_model, tem_model
+
+from open_analysis_amd.fdtd import C0, run, sparams
+from open_analysis_amd.fdtd.materials import fit_djordjevic_sarkar
+from open_analysis_amd.fdtd.model import ModelError
+from open_analysis_amd.fdtd.run import port_voltage_factor, source_spectrum
+from open_analysis_amd.fdtd.subcell import fit_surface_impedance, slab_surface_impedance
+from open_analysis_amd.fdtd.tail import extrapolate_dft, fit_tail
+from oracle import oracle_factory
+
+
+class _ScriptedEngine:
+    """Engine whose energy follows ``energy_of(step)``; no fields."""
+
+    name = "scripted"
+
+    def __init__(self, energy_of: Any) -> None:
+        self._energy_of = energy_of
+        self._n = 0
+
+    def step(self, n: int) -> None:
+        self._n += n
+
+    def energy(self) -> float:
+        return float(self._energy_of(self._n))
+
+    def results(self) -> dict[str, Any]:
+        return {}
+
+    def peak_bytes(self) -> int:
+        return 0
+
+    def series(self) -> dict[str, Any]:
+        return {}
+
+
+def _scripted_run(energy_of: Any, **kw: Any):
+    d = C0 / 40e9 / 20.0
+    m = line_model(80, d, np.array([1e9]), 20, 40, dc=False)
+    eng = _ScriptedEngine(energy_of)
+    return run(m, engine=lambda *a: (eng, {}), dtype="float64", max_steps=20000, **kw)
+
+
+def test_non_finite_energy_is_unstable() -> None:
+    with pytest.raises(ModelError) as ei:
+        _scripted_run(lambda n: float("inf") if n > 640 else 1.0)
+    assert ei.value.code == "FDTD.UNSTABLE"
+
+
+def test_growth_after_excitation_is_unstable() -> None:
+    def energy(n: int) -> float:  # decays, then grows about 1 dB per check long after the pulse
+        return float(np.exp(-n / 2000.0)) if n < 8000 else float(np.exp(-4.0 + (n - 8000) / 200.0))
+
+    with pytest.raises(ModelError) as ei:
+        _scripted_run(energy, end_db=-300.0)
+    assert ei.value.code == "FDTD.UNSTABLE" and "grew" in ei.value.message
+
+
+def test_rise_during_excitation_is_not_flagged() -> None:
+    def energy(n: int) -> float:  # 60 dB rise while the source is on, then decay
+        rise = 1e-6 * 10.0 ** (min(n, 320) / 320.0 * 6.0)
+        return rise * float(np.exp(-max(n - 320, 0) / 200))
+
+    res = _scripted_run(energy, end_db=-40.0)
+    assert res.stopped_by == "end_criterion"
+    # -40 dB below the peak at step 320 is reached at 320 + 200 ln(1e4) = 2162; checks every 64
+    assert res.steps == 64 * int(np.ceil((320 + 200 * np.log(1e4)) / 64))
+
+
+def test_max_steps_without_end_db() -> None:
+    res = _scripted_run(lambda n: 1.0, end_db=None)
+    assert res.stopped_by == "max_steps" and res.steps == 20000
+
+
+# ---- matrix-pencil tail
+
+
+def _damped(t: np.ndarray) -> np.ndarray:
+    return (np.exp(-2e8 * t) * np.cos(2 * np.pi * 3e9 * t + 0.3)
+            + 0.5 * np.exp(-5e8 * t) * np.sin(2 * np.pi * 7e9 * t))
+
+
+def test_tail_continuation_matches_brute_force_dft() -> None:
+    """Two damped tones sampled up to step S: the closed-form DFT of the fitted continuation
+    equals the direct sum of the true samples from S on."""
+    dt, t_first, s_cut = 1e-12, 0.5e-12, 4000
+    t = t_first + np.arange(s_cut) * dt
+    fit = fit_tail(_damped(t), dt, t_first, f_band=10e9)
+    assert fit is not None and fit.residual < 1e-8
+    freqs = np.linspace(1e9, 10e9, 10)
+    t_rest = t_first + np.arange(s_cut, s_cut + 400_000) * dt
+    brute = (np.exp(-2j * np.pi * freqs[:, None] * t_rest[None, :]) * _damped(t_rest)).sum(1) * dt
+    got = extrapolate_dft(fit, dt, t_first + s_cut * dt, freqs)
+    assert np.abs(got - brute).max() < 1e-6 * np.abs(brute).max()
+    # one sample late in the continuation start is a visible error
+    late = extrapolate_dft(fit, dt, t_first + (s_cut + 1) * dt, freqs)
+    assert np.abs(late - brute).max() > 1e-4 * np.abs(brute).max()
+
+
+def test_tail_below_noise_is_skipped() -> None:
+    y = np.zeros(1000)
+    y[10] = 1.0
+    assert fit_tail(y, 1e-12, 0.0, f_band=1e10) is None
+
+
+def _lossy_cavity():
+    from open_analysis_amd.fdtd import Grid, Model, Probe, Region, SoftSource, Waveform
+
+    d = 1e-3
+    g = Grid(np.arange(21) * d, np.arange(17) * d, np.arange(13) * d)
+    m = Model(g, Waveform("gauss_mod", 14e9, 12e9), np.linspace(8e9, 20e9, 25),
+              walls=("pec",) * 6)
+    m.regions.append(Region((-1, -1, -1), (1, 1, 1), 1.0, sigma=0.02))
+    m.sources.append(SoftSource("s", 2, (5 * d, 5 * d, 3 * d), (5 * d, 5 * d, 9 * d)))
+    m.probes.append(Probe("p", "E", 2, (13 * d, 10 * d, 3 * d), (13 * d, 10 * d, 9 * d), "mean"))
+    return m
+
+
+def test_run_tail_extends_a_truncated_dft() -> None:
+    """A lossy PEC cavity stopped at 1500 steps: the matrix-pencil tail brings the probe DFT
+    within 1e-3 of a 12000-step run (7 % off without it)."""
+    m = _lossy_cavity()
+    ref = run(m, engine=oracle_factory, dtype="float64", end_db=None, max_steps=12000).probes["p"]
+    short = run(m, engine=oracle_factory, dtype="float64", end_db=-300.0, max_steps=1500)
+    tailed = run(m, engine=oracle_factory, dtype="float64", end_db=-300.0, max_steps=1500,
+                 tail=True)
+    assert tailed.tail["applied"] and tailed.tail["fits"] == 1
+    scale = np.abs(ref).max()
+    assert np.abs(short.probes["p"] - ref).max() > 0.03 * scale
+    assert np.abs(tailed.probes["p"] - ref).max() < 1e-3 * scale
+
+
+# ---- S-parameter helpers and batching
+
+
+def test_port_voltage_factor_closed_form() -> None:
+    f = np.array([0.0, 1e9, 10e9])
+    np.testing.assert_allclose(port_voltage_factor(1e-12, f).real, np.cos(np.pi * f * 1e-12))
+
+
+def test_source_spectrum_matches_direct_sum() -> None:
+    from open_analysis_amd.fdtd import compile_model
+
+    m = tem_model()
+    c = compile_model(m)
+    got = source_spectrum(c, 700)
+    t = (np.arange(700) + 0.5) * c.dt
+    wave = c.channels[0]["waveform"](t)
+    for fi, f in enumerate(c.frequencies):
+        ref = np.sum(wave * np.exp(-2j * np.pi * f * t)) * c.dt
+        assert abs(got[0, fi] - ref) <= 1e-12 * abs(ref)
+
+
+def test_subset_drive_matches_full_batch() -> None:
+    """Driving only port p2 gives the same results as the p2 member of the full batch."""
+    m = tem_model()
+    full = run(m, engine=oracle_factory, dtype="float64", end_db=None, max_steps=400)
+    sub_m = copy.deepcopy(m)
+    sub_m.excite = ("p2",)
+    sub = run(sub_m, engine=oracle_factory, dtype="float64", end_db=None, max_steps=400)
+    for k, v in sub.probes.items():
+        np.testing.assert_allclose(v[0], full.probes[k][1], rtol=0, atol=1e-12 * np.abs(v).max())
+    s_full, s_sub = sparams(m, full), sparams(sub_m, sub)
+    np.testing.assert_allclose(s_sub[:, 1], s_full[:, 1], rtol=1e-10)
+    assert np.all(np.isnan(s_sub[:, 0]))
+
+
+# ---- material fits
+
+
+def test_debye_fit_bounds() -> None:
+    """Djordjevic-Sarkar laminates fitted over 0.1-40 GHz: eps' within 0.2 %, tan d within 2 %."""
+    for dk, df in [(4.3, 0.02), (3.8, 0.012), (3.5, 0.004), (3.0, 0.0015)]:
+        fit = fit_djordjevic_sarkar(dk, df, 1e9, 0.1e9, 40e9)
+        assert fit.err_eps <= 2e-3 and fit.err_tand <= 2e-2, (dk, df)
+        assert 3 <= len(fit.delta) <= 12 and np.all(fit.delta >= 0)
+
+
+def test_sibc_fit_bounds() -> None:
+    """The R-L ladder reproduces Re Zs of 35 um copper to 0.1 % (smooth) and 3 % (1 um rms)."""
+    smooth = fit_surface_impedance(5.8e7, 35e-6, 1e8, 40e9)
+    assert smooth.err < 1e-3 and smooth.r0 > 0 and np.all(smooth.rp >= 0)
+    f = np.geomspace(1e8, 40e9, 50)
+    zs = slab_surface_impedance(f, 5.8e7, 35e-6)
+    assert np.abs(smooth.eval(f).imag / zs.imag - 1.0).max() < 2e-2
+    z_dc = slab_surface_impedance(np.array([1.0]), 5.8e7, 35e-6)[0].real
+    assert abs(z_dc * 5.8e7 * 35e-6 - 1.0) < 1e-3
+    assert fit_surface_impedance(5.8e7, 35e-6, 1e8, 40e9, rms=1e-6).err < 3e-2
