@@ -112,6 +112,58 @@ def traffic_from_profile(workload: str, dtype: str):
     return data.get(f"{workload}:{dtype}")
 
 
+def bench_batch(a) -> dict:
+    """Many uneven via transitions in one persistent launch per step (seeded geometries,
+    96..192 x 96 x 96..128 cells, 4 excitations each)."""
+    from open_analysis_amd.fdtd import vias
+    from open_analysis_amd.fdtd.batch import HipBatch
+    from open_analysis_amd.fdtd.model import compile_model
+
+    models, specs = vias.geometry_set(a.geometries, seed=11)
+    t0 = time.perf_counter()
+    cs = [compile_model(m) for m in models]
+    t_compile = time.perf_counter() - t0
+    print(f"[bench] compiled {len(cs)} geometries in {t_compile:.1f} s", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    batch = HipBatch(cs, a.dtype, max_steps=a.warmup + a.steps + 8)
+    t_setup = time.perf_counter() - t0
+    batch.step(a.warmup)
+    print(f"[bench] batch ready ({t_setup:.1f} s), warm-up done", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    batch.step(a.steps)
+    wall = time.perf_counter() - t0
+    tm = batch.step_timed(min(a.steps, 64))
+    cells = sum(c.grid.cells * c.batch for c in cs)
+    value = cells * a.steps / wall / 1e6
+    yee_bytes = sum(algorithmic_bytes(c, a.dtype)["yee_step"] for c in cs)
+    yee_avg_s = tm["yee_ms"] / max(tm["launches"], 1) / 1e3
+    achieved = yee_bytes / yee_avg_s / 1e9
+    out = {
+        "metric": "Mcell-updates/s", "value": value, "unit": "Mcell-updates/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": wall / a.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": DT_NAME[a.dtype], "data": "synthetic",
+        "config": {"workload": "via_batch", "geometries": len(cs),
+                   "description": "seeded differential via transitions, 96..192 x 96 x 96..128 "
+                                  "cells at 0.05 mm, 8-cell CPML, 4 ports driven once each",
+                   "grids": [[s.nx, s.ny, s.nz] for s in specs], "cell_updates_per_step": cells,
+                   "parallelism": "single GPU, one persistent launch per step"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "yee_batch",
+                     "bytes_per_launch": yee_bytes, "avg_launch_ms": yee_avg_s * 1e3},
+        "step_breakdown_ms": {"yee_batch": yee_avg_s * 1e3,
+                              "other_kernels_and_gaps": (tm["wall_ms"] - tm["yee_ms"]) / tm["launches"]},
+        "setup_s": {"compile_model": t_compile, "batch": t_setup},
+        "tiling": batch.describe(),
+    }
+    del batch
+    if not a.no_cpu:
+        print("[bench] timing the CPU oracle on geometry 0", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(cs[0], a.cpu_seconds)
+        out["cpu_baseline"]["sample"] += " (geometry 0 of the set)"
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -120,6 +172,7 @@ def main() -> int:
     ap.add_argument("--workload", default="stripline_stack")
     ap.add_argument("--dtype", default="float32", choices=sorted(S_FIELD))
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--geometries", type=int, default=32, help="via_batch: number of geometries")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", type=float, default=0.0, metavar="GIB",
                     help="only run the plain copy kernel over GIB gibibytes and print its rate")
@@ -131,6 +184,9 @@ def main() -> int:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}); the engine has no multi-GPU "
               "path in this build.", file=sys.stderr)
         return 2
+    if a.workload == "via_batch":
+        print(json.dumps(bench_batch(a)), flush=True)
+        return 0
     from open_analysis_amd.fdtd.engine import HipEngine
     from open_analysis_amd.fdtd.model import compile_model
 

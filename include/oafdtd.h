@@ -153,6 +153,28 @@ int oafd_plan(int nx, int ny, int nz, int batch, const int32_t* npml, int waves,
  * kernels: copy_probe (one load in flight per thread) and copy_probe4 (four). out holds 4 values:
  * out[0], out[2] = ms per copy, out[1], out[3] = GB/s counting read + write. */
 int oafd_copy_bandwidth(int64_t bytes, int reps, double* out);
+/* ---- many problems stepped together (no counterpart in the reference, whose engine runs one
+ * compiled problem at a time): uneven grids share one persistent fused-step launch per time
+ * step (yee_batch), one launch each of the face, source and probe kernels, one step counter and
+ * one stream; 64 steps are one graph. The tile shape is chosen by an autotuner over the whole
+ * batch (OAFD_YEE_VARIANT forces one). Each problem may have at most 255 members. Member i is a
+ * borrowed oafd_engine handle: oafd_energy / results / get_field / set_field / peak_bytes /
+ * step_count work on it, oafd_step on it returns OAFD_E_INVALID and oafd_destroy ignores it;
+ * it lives until oafd_batch_destroy. Series recording is not available in a batch. */
+typedef struct oafd_batch oafd_batch;
+int oafd_batch_create(const oafd_desc* const* descs, int n, int dtype, int64_t max_steps, int use_graph,
+                      oafd_batch** out);
+int oafd_batch_step(oafd_batch* b, int64_t n);
+/* eager steps with HIP events: out[0] = summed persistent-launch ms, out[1] = wall ms,
+ * out[2] = launches, out[3] = summed ms between launches */
+int oafd_batch_step_timed(oafd_batch* b, int64_t n, double* out);
+int oafd_batch_count(const oafd_batch* b);
+int64_t oafd_batch_step_count(const oafd_batch* b);
+oafd_engine* oafd_batch_member(oafd_batch* b, int i);
+/* JSON text: tile shape, tile count, persistent grid and the autotuner's timings */
+int oafd_batch_describe(const oafd_batch* b, char* buf, int len);
+void oafd_batch_destroy(oafd_batch* b);
+
 /* e == NULL: the next oafd_create returns `code` without touching the device (CPU tests);
  * otherwise the next call on e returns `code`. */
 int oafd_debug_inject_fault(oafd_engine* e, int code);

@@ -28,10 +28,8 @@ struct SibcArgs {
 };
 
 template <typename S, typename R>
-__global__ __launch_bounds__(256) void sibc_step(SibcArgs<R> A) {
-    const int e = blockIdx.x * 256 + threadIdx.x;
-    const int b = blockIdx.y;
-    if (e >= A.n) return;
+__device__ __forceinline__ void sibc_body(const SibcArgs<R>& A, int e, int b) {
+    if (e >= A.n || b >= A.B) return;
     const long long bb = (long long)b * A.bs;
     R jv = (R)0;
 #pragma unroll
@@ -51,6 +49,17 @@ __global__ __launch_bounds__(256) void sibc_step(SibcArgs<R> A) {
     }
     A.jprev[be] = jv;
     stS<S, R>(A.E[A.comp[e]], bb + A.flat[e], out);
+}
+
+template <typename S, typename R>
+__global__ __launch_bounds__(256) void sibc_step(SibcArgs<R> A) {
+    sibc_body<S, R>(A, blockIdx.x * 256 + threadIdx.x, blockIdx.y);
+}
+
+// many problems: grid (edges / 256, members, problems)
+template <typename S, typename R>
+__global__ __launch_bounds__(256) void sibc_multi(const SibcArgs<R>* __restrict__ all) {
+    sibc_body<S, R>(all[blockIdx.z], blockIdx.x * 256 + threadIdx.x, blockIdx.y);
 }
 
 // ---- sources (yee.cu:96-108): one thread per unique target edge, terms in channel order.
@@ -80,10 +89,9 @@ struct InjectArgs {
 };
 
 template <typename S, typename R>
-__global__ __launch_bounds__(256) void inject_step(InjectArgs<R> A) {
-    const int t = blockIdx.x * 256 + threadIdx.x;
+__device__ __forceinline__ void inject_body(const InjectArgs<R>& A, int t, int slot) {
     if (t >= A.n) return;
-    const long long step = *A.step_base + A.step_off;
+    const long long step = *A.step_base + slot;
     const int c = A.comp[t];
     const int f = A.flat[t];
     const int m = A.idx[c][f];
@@ -113,6 +121,17 @@ __global__ __launch_bounds__(256) void inject_step(InjectArgs<R> A) {
     }
 }
 
+template <typename S, typename R>
+__global__ __launch_bounds__(256) void inject_step(InjectArgs<R> A) {
+    inject_body<S, R>(A, blockIdx.x * 256 + threadIdx.x, A.step_off);
+}
+
+// many problems: grid (targets / 256, problems)
+template <typename S, typename R>
+__global__ __launch_bounds__(256) void inject_multi(const InjectArgs<R>* __restrict__ all, int slot) {
+    inject_body<S, R>(all[blockIdx.y], blockIdx.x * 256 + threadIdx.x, slot);
+}
+
 // ---- probes (yee.cu:112-131): weighted sums in FP64, fixed-order reduction, grid (P, B)
 struct ProbeArgs {
     int nE, nH, B;
@@ -130,9 +149,8 @@ struct ProbeArgs {
 };
 
 template <typename S>
-__global__ __launch_bounds__(256) void probe_step(ProbeArgs A) {
-    __shared__ double red[256];
-    const int p = blockIdx.x, b = blockIdx.y;
+__device__ __forceinline__ void probe_body(const ProbeArgs& A, int p, int b, int slot, double (&red)[256]) {
+    if (p >= A.nE + A.nH || b >= A.B) return;  // uniform over the workgroup
     const int c = A.comp[p];
     const long long bb = (long long)b * A.bs;
     double acc = 0.0;
@@ -146,10 +164,23 @@ __global__ __launch_bounds__(256) void probe_step(ProbeArgs A) {
     }
     if (threadIdx.x == 0) {
         if (p < A.nE)
-            A.ringE[((long long)A.slot * A.B + b) * A.nE + p] = red[0] * A.inv_fs;
+            A.ringE[((long long)slot * A.B + b) * A.nE + p] = red[0] * A.inv_fs;
         else
-            A.ringH[((long long)A.slot * A.B + b) * A.nH + (p - A.nE)] = red[0] * A.inv_hs;
+            A.ringH[((long long)slot * A.B + b) * A.nH + (p - A.nE)] = red[0] * A.inv_hs;
     }
+}
+
+template <typename S>
+__global__ __launch_bounds__(256) void probe_step(ProbeArgs A) {
+    __shared__ double red[256];
+    probe_body<S>(A, blockIdx.x, blockIdx.y, A.slot, red);
+}
+
+// many problems: grid (probes, members, problems)
+template <typename S>
+__global__ __launch_bounds__(256) void probe_multi(const ProbeArgs* __restrict__ all, int slot) {
+    __shared__ double red[256];
+    probe_body<S>(all[blockIdx.z], blockIdx.x, blockIdx.y, slot, red);
 }
 
 // ---- running DFT with Kahan-compensated FP64 accumulation: one thread per (f, b*P + p)

@@ -75,6 +75,7 @@ struct oafd_engine {
     int64_t peak = 0;
     int64_t cur_bytes = 0;
     int pending_fault = 0;
+    bool in_batch = false;  // a member of an oafd_batch: stepped and destroyed by the batch
     std::string err;
 };
 
@@ -88,13 +89,15 @@ struct YeeVariant {
     const void* interior;
     const void* general;
     const void* zslab;
+    const void* batch;  // persistent kernel over the tiles of many problems
 };
 
 template <typename S, typename R, int NW, int RPW>
 YeeVariant<R> variant() {
     return {NW, RPW, reinterpret_cast<const void*>(&oafd::yee_fused<S, R, NW, RPW, false, false>),
             reinterpret_cast<const void*>(&oafd::yee_fused<S, R, NW, RPW, true, false>),
-            reinterpret_cast<const void*>(&oafd::yee_fused<S, R, NW, RPW, false, true>)};
+            reinterpret_cast<const void*>(&oafd::yee_fused<S, R, NW, RPW, false, true>),
+            reinterpret_cast<const void*>(&oafd::yee_batch<S, R, NW, RPW>)};
 }
 
 template <typename S, typename R>
@@ -197,6 +200,7 @@ struct Engine final : oafd_engine {
     bool record_series;
     bool use_graph;
     hipStream_t stream = nullptr;
+    bool own_stream = true;  // false once a batch has given the engine its stream
     std::vector<void*> allocs;
     // fields [parity][comp]
     void* E[2][3] = {};
@@ -254,10 +258,10 @@ struct Engine final : oafd_engine {
         if (graph) (void)hipGraphDestroy(graph);
         if (stream) (void)hipStreamSynchronize(stream);
         for (void* p : allocs) (void)hipFree(p);
-        if (stream) (void)hipStreamDestroy(stream);
+        if (stream && own_stream) (void)hipStreamDestroy(stream);
     }
 
-    void init(const oafd_desc* d, bool rec, int64_t maxs, bool graph_on) {
+    void init(const oafd_desc* d, bool rec, int64_t maxs, bool graph_on, bool autotune = true) {
         nx = d->nx; ny = d->ny; nz = d->nz;
         X = d->X; Y = d->Y; Z = d->Z; B = d->batch;
         bs = (long long)X * Y * Z;
@@ -490,7 +494,12 @@ struct Engine final : oafd_engine {
         scalar = dev<double>(1);
         ena.partial = partial;
         HIP_TRY(hipDeviceSynchronize());
-        tune();
+        if (autotune) {
+            tune();
+        } else {
+            vars = variants<S, R>();
+            plan = make_plan(0);
+        }
     }
 
     // ---- fused-step planning: z chunks aligned with the CPML slabs, interior box, general list
@@ -500,8 +509,7 @@ struct Engine final : oafd_engine {
         P.nck = nck;
         P.nw = vars[v].nw;
         P.rpw = vars[v].rpw;
-        const bool zslab_ok = !base.pmc_lo[2] && !base.pmc_hi[2];
-        const HostPlan H = plan_tiles(nx, ny, nz, B, npml_, P.nw, P.rpw, zslab_ok, nck);
+        const HostPlan H = plan_tiles(nx, ny, nz, B, npml_, P.nw, P.rpw, zslab_ok(), nck);
         P.n_chunks = (int)H.kb.size() - 1;
         P.kb = dev_vec(H.kb);
         P.n_gen = (int)H.gen.size();
@@ -523,7 +531,8 @@ struct Engine final : oafd_engine {
         return P;
     }
 
-    void launch_yee(int parity) {
+    // Arguments of the fused step that reads field set `parity` and writes the other.
+    oafd::YeeArgs<R> yee_args(int parity) const {
         oafd::YeeArgs<R> A = base;
         for (int c = 0; c < 3; ++c) {
             A.Ein[c] = E[parity][c];
@@ -539,6 +548,11 @@ struct Engine final : oafd_engine {
         A.ty0 = plan.ty0;
         A.nxi = plan.nxi;
         A.nyi = plan.nyi;
+        return A;
+    }
+    bool zslab_ok() const { return !base.pmc_lo[2] && !base.pmc_hi[2]; }
+    void launch_yee(int parity) {
+        oafd::YeeArgs<R> A = yee_args(parity);
         const dim3 block(oafd::TXH, plan.nw);
         const YeeVariant<R>& v = vars[plan.variant];
         auto go = [&](const void* fn, int ntiles) {
@@ -637,31 +651,44 @@ struct Engine final : oafd_engine {
         s += "]}";
         return s;
     }
+    // Face, source and probe arguments after the fused step that wrote field set `out`.
+    oafd::SibcArgs<R> sibc_args(int out) const {
+        oafd::SibcArgs<R> a = sib;
+        for (int c = 0; c < 3; ++c) {
+            a.E[c] = E[out][c];
+            a.H[c] = H[out][c];
+        }
+        return a;
+    }
+    oafd::InjectArgs<R> inject_args(int out, int slot) const {
+        oafd::InjectArgs<R> a = inj;
+        for (int c = 0; c < 3; ++c) a.E[c] = E[out][c];
+        a.step_off = slot;
+        return a;
+    }
+    oafd::ProbeArgs probe_args(int out, int slot) const {
+        oafd::ProbeArgs a = prb;
+        for (int c = 0; c < 3; ++c) {
+            a.F[c] = E[out][c];
+            a.F[3 + c] = H[out][c];
+        }
+        a.slot = slot;
+        return a;
+    }
     void launch_rest(int slot, int parity) {
         const int out = parity ^ 1;
         if (have_sibc) {
-            oafd::SibcArgs<R> a = sib;
-            for (int c = 0; c < 3; ++c) {
-                a.E[c] = E[out][c];
-                a.H[c] = H[out][c];
-            }
+            const oafd::SibcArgs<R> a = sibc_args(out);
             hipLaunchKernelGGL((oafd::sibc_step<S, R>), dim3((a.n + 255) / 256, B), dim3(256), 0, stream, a);
             HIP_TRY(hipGetLastError());
         }
         if (have_inj) {
-            oafd::InjectArgs<R> a = inj;
-            for (int c = 0; c < 3; ++c) a.E[c] = E[out][c];
-            a.step_off = slot;
+            const oafd::InjectArgs<R> a = inject_args(out, slot);
             hipLaunchKernelGGL((oafd::inject_step<S, R>), dim3((a.n + 255) / 256), dim3(256), 0, stream, a);
             HIP_TRY(hipGetLastError());
         }
         if (have_probe) {
-            oafd::ProbeArgs a = prb;
-            for (int c = 0; c < 3; ++c) {
-                a.F[c] = E[out][c];
-                a.F[3 + c] = H[out][c];
-            }
-            a.slot = slot;
+            const oafd::ProbeArgs a = probe_args(out, slot);
             hipLaunchKernelGGL((oafd::probe_step<S>), dim3(nP[0] + nP[1], B), dim3(256), 0, stream, a);
             HIP_TRY(hipGetLastError());
         }
@@ -734,6 +761,7 @@ struct Engine final : oafd_engine {
     }
 
     int step(int64_t n) override {
+        if (in_batch) return fail(OAFD_E_INVALID, "a batch member is stepped through its batch");
         if (n < 0) return fail(OAFD_E_INVALID, "negative step count");
         if (step_count + n > max_steps) return fail(OAFD_E_MAX_STEPS, "step count exceeds max_steps (wave table length)");
         int64_t done = 0;
@@ -758,6 +786,7 @@ struct Engine final : oafd_engine {
     }
 
     int step_timed(int64_t n, double* out) override {
+        if (in_batch) return fail(OAFD_E_INVALID, "a batch member is stepped through its batch");
         if (step_count + n > max_steps) return fail(OAFD_E_MAX_STEPS, "step count exceeds max_steps (wave table length)");
         std::vector<hipEvent_t> ev(2 * n + 2);
         for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
@@ -874,6 +903,393 @@ struct Engine final : oafd_engine {
         HIP_TRY(e2);
         HIP_TRY(e3);
         return OAFD_OK;
+    }
+};
+
+// ---- many problems in one persistent launch per step
+//
+// The members are complete engines (their own fields, tables, probes, DFT) that share one
+// stream and one step counter. Every step runs one yee_batch launch over the tiles of all
+// members, then one launch each of the face, source and probe kernels over all members; 64
+// steps are captured as one graph. A member tile runs the same yee_tile code as the
+// single-problem kernels, so each member's results do not depend on the rest of the batch.
+struct BatchBase {
+    virtual ~BatchBase() = default;
+    virtual int step(int64_t n) = 0;
+    virtual int step_timed(int64_t n, double* out) = 0;
+    virtual std::string describe() const = 0;
+    virtual oafd_engine* member(int i) = 0;
+    int count = 0;
+    int64_t step_count = 0;
+    int64_t max_steps = 0;
+};
+
+template <typename S, typename R>
+struct Batch final : BatchBase {
+    std::vector<Engine<S, R>*> mem;
+    hipStream_t stream = nullptr;
+    std::vector<void*> allocs;
+    std::vector<YeeVariant<R>> vars;
+    int variant = 0, nck = 0, grid = 0, nlist = 0;
+    oafd::YeeArgs<R>* dargs[2] = {nullptr, nullptr};
+    int4* list = nullptr;
+    int* xs = nullptr;
+    unsigned* ctr = nullptr;
+    long long* step_base = nullptr;
+    oafd::SibcArgs<R>* dsib[2] = {nullptr, nullptr};
+    oafd::InjectArgs<R>* dinj[2] = {nullptr, nullptr};
+    oafd::ProbeArgs* dprb[2] = {nullptr, nullptr};
+    int nsib = 0, max_sib = 0, ninj = 0, max_inj = 0, nprb = 0, max_prb = 0, maxB = 0;
+    bool use_graph = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    int graph_state = 0;
+    struct Trial {
+        int variant, nck, tiles;
+        double ms;
+    };
+    std::vector<Trial> tried;
+
+    template <typename T>
+    T* dev(size_t count, const T* host = nullptr) {
+        if (count == 0) count = 1;
+        void* p = nullptr;
+        HIP_TRY(hipMalloc(&p, count * sizeof(T)));
+        allocs.push_back(p);
+        if (host) {
+            HIP_TRY(hipMemcpy(p, host, count * sizeof(T), hipMemcpyHostToDevice));
+        } else {
+            HIP_TRY(hipMemset(p, 0, count * sizeof(T)));
+        }
+        return static_cast<T*>(p);
+    }
+
+    ~Batch() override {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (auto* m : mem) delete m;
+        for (void* p : allocs) (void)hipFree(p);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    oafd_engine* member(int i) override { return (i >= 0 && i < count) ? mem[i] : nullptr; }
+
+    void init(const oafd_desc* const* descs, int n, int64_t maxs, bool graph_on) {
+        count = n;
+        max_steps = maxs;
+        use_graph = graph_on;
+        HIP_TRY(hipStreamCreate(&stream));
+        step_base = dev<long long>(1);
+        long long cells = 0;
+        for (int i = 0; i < n; ++i) {
+            auto* m = new Engine<S, R>();
+            mem.push_back(m);
+            m->init(descs[i], false, maxs, false, false);
+            m->in_batch = true;
+            HIP_TRY(hipStreamSynchronize(m->stream));
+            HIP_TRY(hipStreamDestroy(m->stream));
+            m->stream = stream;
+            m->own_stream = false;
+            m->step_base = step_base;
+            m->inj.step_base = step_base;
+            maxB = std::max(maxB, m->B);
+            cells += (long long)m->bs * m->B;
+        }
+        vars = variants<S, R>();
+        // face, source and probe arguments of every member, per parity of the written set
+        for (int p = 0; p < 2; ++p) {
+            std::vector<oafd::SibcArgs<R>> sa;
+            std::vector<oafd::InjectArgs<R>> ia;
+            std::vector<oafd::ProbeArgs> pa;
+            for (auto* m : mem) {
+                if (m->have_sibc) sa.push_back(m->sibc_args(p ^ 1));
+                if (m->have_inj) ia.push_back(m->inject_args(p ^ 1, 0));
+                if (m->have_probe) pa.push_back(m->probe_args(p ^ 1, 0));
+            }
+            nsib = (int)sa.size();
+            ninj = (int)ia.size();
+            nprb = (int)pa.size();
+            for (auto& a : sa) max_sib = std::max(max_sib, a.n);
+            for (auto& a : ia) max_inj = std::max(max_inj, a.n);
+            for (auto& a : pa) max_prb = std::max(max_prb, a.nE + a.nH);
+            if (nsib) dsib[p] = dev<oafd::SibcArgs<R>>(sa.size(), sa.data());
+            if (ninj) dinj[p] = dev<oafd::InjectArgs<R>>(ia.size(), ia.data());
+            if (nprb) dprb[p] = dev<oafd::ProbeArgs>(pa.size(), pa.data());
+        }
+        ctr = dev<unsigned>(2 * oafd::NXCD);
+        tune(cells);
+    }
+
+    // Tile list of every member for tile shape v and interior chunk count nck (0 = default).
+    void build(int v, int nck_req) {
+        const int nw = vars[v].nw, rpw = vars[v].rpw;
+        std::vector<std::vector<int4>> part(oafd::NXCD);
+        std::vector<oafd::YeeArgs<R>> args[2];
+        for (int mi = 0; mi < count; ++mi) {
+            Engine<S, R>& m = *mem[mi];
+            m.plan = m.make_plan(v, nck_req);
+            const HostPlan H = plan_tiles(m.nx, m.ny, m.nz, m.B, m.npml_, nw, rpw, m.zslab_ok(), nck_req);
+            std::vector<int4> ent;
+            auto add = [&](int kind, int tx, int ty, int zc, int b) {
+                ent.push_back(make_int4(tx | (kind << 16), ty, zc, b | (mi << 8)));
+            };
+            for (int b = 0; b < m.B; ++b) {
+                for (int q = 0; q < H.nzc; ++q)
+                    for (int tx = H.tx0; tx < H.tx0 + H.nxi; ++tx)
+                        for (int ty = H.ty0; ty < H.ty0 + H.nyi; ++ty) add(0, tx, ty, H.zc0 + q, b);
+                for (int zc : H.zs)
+                    for (int tx = H.tx0; tx < H.tx0 + H.nxi; ++tx)
+                        for (int ty = H.ty0; ty < H.ty0 + H.nyi; ++ty) add(1, tx, ty, zc, b);
+            }
+            for (const int4& g : H.gen) add(2, g.x, g.y, g.z, g.w);
+            // member order (b, chunk, tx, ty), ty fastest: neighbours in y share E halo rows
+            std::sort(ent.begin(), ent.end(), [](const int4& a, const int4& b) {
+                const int ba = a.w & 0xff, bb = b.w & 0xff;
+                if (ba != bb) return ba < bb;
+                if (a.z != b.z) return a.z < b.z;
+                if ((a.x & 0xffff) != (b.x & 0xffff)) return (a.x & 0xffff) < (b.x & 0xffff);
+                return a.y < b.y;
+            });
+            // one contiguous eighth of every member per XCD queue
+            const size_t len = ent.size();
+            for (int x = 0; x < oafd::NXCD; ++x)
+                part[x].insert(part[x].end(), ent.begin() + len * x / oafd::NXCD,
+                               ent.begin() + len * (x + 1) / oafd::NXCD);
+            for (int p = 0; p < 2; ++p) args[p].push_back(m.yee_args(p));
+        }
+        std::vector<int4> all;
+        std::vector<int> start(oafd::NXCD + 1, 0);
+        for (int x = 0; x < oafd::NXCD; ++x) {
+            start[x] = (int)all.size();
+            all.insert(all.end(), part[x].begin(), part[x].end());
+        }
+        start[oafd::NXCD] = (int)all.size();
+        nlist = (int)all.size();
+        list = dev<int4>(all.size(), all.data());
+        xs = dev<int>(start.size(), start.data());
+        for (int p = 0; p < 2; ++p) dargs[p] = dev<oafd::YeeArgs<R>>(args[p].size(), args[p].data());
+        variant = v;
+        nck = nck_req;
+        // persistent grid: the resident workgroups of the device, a multiple of the XCD count
+        int dev_id = 0, cus = 0, per_cu = 0;
+        HIP_TRY(hipGetDevice(&dev_id));
+        HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev_id));
+        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, vars[v].batch, oafd::TXH * nw, 0));
+        const int resident = std::max(1, cus * std::max(per_cu, 1));
+        const int want = std::min(resident, std::max(nlist, 1));
+        grid = oafd::NXCD * ((want + oafd::NXCD - 1) / oafd::NXCD);
+    }
+
+    void launch_yee(int parity) {
+        void* a[] = {&dargs[parity], &list, &xs, &ctr, &parity};
+        HIP_TRY(hipLaunchKernel(vars[variant].batch, dim3(grid), dim3(oafd::TXH, vars[variant].nw), a, 0, stream));
+    }
+    void launch_rest(int slot, int parity) {
+        if (nsib) {
+            hipLaunchKernelGGL((oafd::sibc_multi<S, R>), dim3((max_sib + 255) / 256, maxB, nsib), dim3(256), 0, stream,
+                               dsib[parity]);
+            HIP_TRY(hipGetLastError());
+        }
+        if (ninj) {
+            hipLaunchKernelGGL((oafd::inject_multi<S, R>), dim3((max_inj + 255) / 256, ninj), dim3(256), 0, stream,
+                               dinj[parity], slot);
+            HIP_TRY(hipGetLastError());
+        }
+        if (nprb) {
+            hipLaunchKernelGGL((oafd::probe_multi<S>), dim3(max_prb, maxB, nprb), dim3(256), 0, stream, dprb[parity],
+                               slot);
+            HIP_TRY(hipGetLastError());
+        }
+    }
+    void launch_chunk_end(int slot0, int count_) {
+        for (auto* m : mem) m->launch_dft(slot0, count_);
+        hipLaunchKernelGGL(oafd::advance_step, dim3(1), dim3(64), 0, stream, step_base, CHUNK);
+        HIP_TRY(hipGetLastError());
+    }
+    void reset_counters() { HIP_TRY(hipMemsetAsync(ctr, 0, 2 * oafd::NXCD * sizeof(unsigned), stream)); }
+
+    // Autotuner over tile shapes and interior chunk counts, timing the persistent launch alone.
+    void tune(long long cells) {
+        const char* force = std::getenv("OAFD_YEE_VARIANT");
+        if (force && *force) {
+            build(std::min<int>(std::max(0, std::atoi(force)), (int)vars.size() - 1), 0);
+        } else if (cells < (1LL << 22)) {
+            build(0, 0);
+        } else {
+            hipEvent_t e0, e1;
+            HIP_TRY(hipEventCreate(&e0));
+            HIP_TRY(hipEventCreate(&e1));
+            int best = -1;
+            static const int chunk_counts[] = {0, 2, 4};
+            for (int v = 0; v < (int)vars.size(); ++v)
+                for (int c : chunk_counts) {
+                    build(v, c);
+                    reset_counters();
+                    for (int s = 0; s < 2; ++s) launch_yee(s & 1);
+                    HIP_TRY(hipEventRecord(e0, stream));
+                    for (int s = 0; s < 4; ++s) launch_yee(s & 1);
+                    HIP_TRY(hipEventRecord(e1, stream));
+                    HIP_TRY(hipEventSynchronize(e1));
+                    float ms = 0.f;
+                    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+                    tried.push_back({v, c, nlist, ms / 4.0});
+                    if (best < 0 || tried.back().ms < tried[best].ms) best = (int)tried.size() - 1;
+                }
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            build(tried[best].variant, tried[best].nck);
+            for (auto* m : mem) {  // the timed steps ran on zero fields; clear every state anyway
+                const size_t nfield = (size_t)m->B * m->bs * sizeof(S);
+                for (int p = 0; p < 2; ++p)
+                    for (int c = 0; c < 3; ++c) {
+                        HIP_TRY(hipMemsetAsync(m->E[p][c], 0, nfield, stream));
+                        HIP_TRY(hipMemsetAsync(m->H[p][c], 0, nfield, stream));
+                    }
+                HIP_TRY(hipStreamSynchronize(stream));
+                m->zero_state();
+            }
+        }
+        reset_counters();
+        HIP_TRY(hipStreamSynchronize(stream));
+    }
+
+    void capture() {
+        hipError_t e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) {
+            graph_state = -1;
+            (void)hipGetLastError();
+            return;
+        }
+        bool ok = true;
+        try {
+            for (int s = 0; s < CHUNK; ++s) {
+                launch_yee(s & 1);
+                launch_rest(s, s & 1);
+            }
+            launch_chunk_end(0, CHUNK);
+        } catch (const HipFail&) {
+            ok = false;
+        }
+        hipGraph_t g = nullptr;
+        e = hipStreamEndCapture(stream, &g);
+        if (!ok || e != hipSuccess || g == nullptr) {
+            graph_state = -1;
+            if (g) (void)hipGraphDestroy(g);
+            (void)hipGetLastError();
+            return;
+        }
+        if (hipGraphInstantiate(&graph_exec, g, nullptr, nullptr, 0) != hipSuccess) {
+            graph_state = -1;
+            (void)hipGraphDestroy(g);
+            (void)hipGetLastError();
+            return;
+        }
+        graph = g;
+        graph_state = 1;
+    }
+
+    void advance_members(int take) {
+        step_count += take;
+        for (auto* m : mem) m->step_count = step_count;
+    }
+
+    int step(int64_t n) override {
+        if (n < 0) return fail(OAFD_E_INVALID, "negative step count");
+        if (step_count + n > max_steps) return fail(OAFD_E_MAX_STEPS, "step count exceeds max_steps (wave table length)");
+        int64_t done = 0;
+        while (done < n) {
+            const int64_t start = step_count;
+            const int in_chunk = (int)(start % CHUNK);
+            const int take = (int)std::min<int64_t>(n - done, CHUNK - in_chunk);
+            if (use_graph && take == CHUNK && graph_state == 0) capture();
+            if (graph_state == 1 && take == CHUNK) {
+                HIP_TRY(hipGraphLaunch(graph_exec, stream));
+            } else {
+                for (int s = 0; s < take; ++s) {
+                    const int parity = (int)((start + s) & 1);
+                    launch_yee(parity);
+                    launch_rest(in_chunk + s, parity);
+                }
+                for (auto* m : mem) m->launch_dft(in_chunk, take);
+                if (in_chunk + take == CHUNK) {
+                    hipLaunchKernelGGL(oafd::advance_step, dim3(1), dim3(64), 0, stream, step_base, CHUNK);
+                    HIP_TRY(hipGetLastError());
+                }
+            }
+            advance_members(take);
+            done += take;
+        }
+        HIP_TRY(hipStreamSynchronize(stream));
+        return OAFD_OK;
+    }
+
+    // Eager steps with HIP events around every persistent launch: out[0] = summed launch ms,
+    // out[1] = wall ms, out[2] = launches, out[3] = summed ms between launches.
+    int step_timed(int64_t n, double* out) override {
+        if (n < 0) return fail(OAFD_E_INVALID, "negative step count");
+        if (step_count + n > max_steps) return fail(OAFD_E_MAX_STEPS, "step count exceeds max_steps (wave table length)");
+        std::vector<hipEvent_t> ev(2 * n + 2);
+        for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
+        size_t evi = 1;
+        HIP_TRY(hipEventRecord(ev[0], stream));
+        int64_t done = 0;
+        while (done < n) {
+            const int64_t start = step_count;
+            const int in_chunk = (int)(start % CHUNK);
+            const int take = (int)std::min<int64_t>(n - done, CHUNK - in_chunk);
+            for (int s = 0; s < take; ++s) {
+                const int parity = (int)((start + s) & 1);
+                HIP_TRY(hipEventRecord(ev[evi], stream));
+                launch_yee(parity);
+                HIP_TRY(hipEventRecord(ev[evi + 1], stream));
+                evi += 2;
+                launch_rest(in_chunk + s, parity);
+            }
+            for (auto* m : mem) m->launch_dft(in_chunk, take);
+            if (in_chunk + take == CHUNK) {
+                hipLaunchKernelGGL(oafd::advance_step, dim3(1), dim3(64), 0, stream, step_base, CHUNK);
+                HIP_TRY(hipGetLastError());
+            }
+            advance_members(take);
+            done += take;
+        }
+        HIP_TRY(hipEventRecord(ev[evi], stream));
+        HIP_TRY(hipEventSynchronize(ev[evi]));
+        double yee = 0.0, rest = 0.0;
+        float ms = 0.f;
+        for (size_t q = 1; q + 1 < evi; q += 2) {
+            HIP_TRY(hipEventElapsedTime(&ms, ev[q], ev[q + 1]));
+            yee += ms;
+            if (q + 2 < evi) {
+                HIP_TRY(hipEventElapsedTime(&ms, ev[q + 1], ev[q + 2]));
+                rest += ms;
+            }
+        }
+        HIP_TRY(hipEventElapsedTime(&ms, ev[0], ev[evi]));
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        out[0] = yee;
+        out[1] = ms;
+        out[2] = (double)n;
+        out[3] = rest;
+        return OAFD_OK;
+    }
+
+    std::string describe() const override {
+        char buf[256];
+        std::snprintf(buf, sizeof buf,
+                      "{\"problems\": %d, \"waves\": %d, \"rows_per_wave\": %d, \"chunks_requested\": %d, "
+                      "\"tiles\": %d, \"grid\": %d, \"tuning_ms\": [",
+                      count, vars[variant].nw, vars[variant].rpw, nck, nlist, grid);
+        std::string s = buf;
+        for (size_t q = 0; q < tried.size(); ++q) {
+            std::snprintf(buf, sizeof buf, "%s{\"waves\": %d, \"rows_per_wave\": %d, \"chunks_requested\": %d, "
+                          "\"tiles\": %d, \"ms\": %.5f}", q ? ", " : "", vars[tried[q].variant].nw,
+                          vars[tried[q].variant].rpw, tried[q].nck, tried[q].tiles, tried[q].ms);
+            s += buf;
+        }
+        s += "]}";
+        return s;
     }
 };
 
@@ -1107,6 +1523,92 @@ const char* oafd_last_error(const oafd_engine* e) {
     return g_err.c_str();
 }
 
-void oafd_destroy(oafd_engine* e) { delete e; }
+void oafd_destroy(oafd_engine* e) {
+    if (e && e->in_batch) return;  // owned by its batch
+    delete e;
+}
+
+struct oafd_batch {
+    std::unique_ptr<BatchBase> impl;
+};
+
+int oafd_batch_create(const oafd_desc* const* descs, int n, int dtype, int64_t max_steps, int use_graph,
+                      oafd_batch** out) {
+    if (!out) return fail(OAFD_E_INVALID, "null output pointer");
+    *out = nullptr;
+    if (!descs || n < 1 || n >= (1 << 23)) return fail(OAFD_E_INVALID, "need 1 .. 2^23 - 1 problems");
+    for (int i = 0; i < n; ++i) {
+        const int rc = check_desc(descs[i], dtype);
+        if (rc) return rc;
+        if (descs[i]->batch > 255) return fail(OAFD_E_INVALID, "a batched problem may have at most 255 members");
+        if ((descs[i]->nx + 1 + oafd::TXE - 1) / oafd::TXE >= (1 << 16)) return fail(OAFD_E_INVALID, "grid too wide");
+    }
+    if (max_steps < 1) return fail(OAFD_E_INVALID, "max_steps must be positive");
+    if (oafd_device_count() < 1) return fail(OAFD_E_NO_DEVICE, "no ROCm device visible");
+    auto* b = new (std::nothrow) oafd_batch();
+    if (!b) return fail(OAFD_E_OOM, "host allocation failed");
+    try {
+        switch (dtype) {
+            case OAFD_F32: {
+                auto* x = new Batch<float, float>();
+                b->impl.reset(x);
+                x->init(descs, n, max_steps, use_graph != 0);
+                break;
+            }
+            case OAFD_F64: {
+                auto* x = new Batch<double, double>();
+                b->impl.reset(x);
+                x->init(descs, n, max_steps, use_graph != 0);
+                break;
+            }
+            case OAFD_F16: {
+                auto* x = new Batch<_Float16, float>();
+                b->impl.reset(x);
+                x->init(descs, n, max_steps, use_graph != 0);
+                break;
+            }
+            default: {
+                auto* x = new Batch<__bf16, float>();
+                b->impl.reset(x);
+                x->init(descs, n, max_steps, use_graph != 0);
+                break;
+            }
+        }
+    } catch (const HipFail& f) {
+        delete b;
+        return fail(f.code, f.msg);
+    } catch (const std::bad_alloc&) {
+        delete b;
+        return fail(OAFD_E_OOM, "host allocation failed");
+    }
+    *out = b;
+    return OAFD_OK;
+}
+
+#define BATCH_CALL(b, body)                                       \
+    do {                                                          \
+        if (!(b) || !(b)->impl) return fail(OAFD_E_INVALID, "null batch"); \
+        try {                                                     \
+            body;                                                 \
+        } catch (const HipFail& f) {                              \
+            return fail(f.code, f.msg);                           \
+        }                                                         \
+    } while (0)
+
+int oafd_batch_step(oafd_batch* b, int64_t n) { BATCH_CALL(b, return b->impl->step(n)); }
+int oafd_batch_step_timed(oafd_batch* b, int64_t n, double* out) {
+    if (!out) return fail(OAFD_E_INVALID, "null output");
+    BATCH_CALL(b, return b->impl->step_timed(n, out));
+}
+int oafd_batch_count(const oafd_batch* b) { return (b && b->impl) ? b->impl->count : 0; }
+int64_t oafd_batch_step_count(const oafd_batch* b) { return (b && b->impl) ? b->impl->step_count : 0; }
+oafd_engine* oafd_batch_member(oafd_batch* b, int i) { return (b && b->impl) ? b->impl->member(i) : nullptr; }
+int oafd_batch_describe(const oafd_batch* b, char* buf, int len) {
+    if (!b || !b->impl || !buf || len <= 0) return fail(OAFD_E_INVALID, "null batch or buffer");
+    const std::string s = b->impl->describe();
+    std::snprintf(buf, len, "%s", s.c_str());
+    return (int)s.size() < len ? OAFD_OK : fail(OAFD_E_INVALID, "describe buffer too small");
+}
+void oafd_batch_destroy(oafd_batch* b) { delete b; }
 
 }  // extern "C"

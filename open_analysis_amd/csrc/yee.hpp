@@ -109,30 +109,13 @@ __device__ __forceinline__ void cpml_term(R& d, const Slab<R>& sl, const R* pin,
     d = d + sl.k * d + psi;
 }
 
+// One tile: E cells [1 + 63 tx, +63) x [1 + (TYH-1) ty, +TYH-1) of member b, z chunk zc.
+// sE / sH are the workgroup's LDS slots (E^n(k) with its high halo; H^{n+1/2}(k)).
 template <typename S, typename R, int NW, int RPW, bool GEN, bool ZS>
-__global__ __launch_bounds__(TXH * NW) void yee_fused(YeeArgs<R> A) {
-    // GEN: general tiles. ZS (interior tiles of the first and last z chunks): x/y interior, z CPML
-    // slab applied per plane, no wall ghosts (z walls are CPML or PEC).
+__device__ __forceinline__ void yee_tile(const YeeArgs<R>& A, int tx, int ty, int zc, int b,
+                                         R (&sE)[3][NW * RPW + 1][TXH + 1], R (&sH)[3][NW * RPW][TXH]) {
     constexpr int TYH = NW * RPW;  // H rows per tile
     constexpr int TYE = TYH - 1;   // E rows per tile
-    __shared__ R sE[3][TYH + 1][TXH + 1];
-    __shared__ R sH[3][TYH][TXH];
-
-    // ---- XCD-aware tile: XCD x runs the contiguous tile range [x*per, (x+1)*per)
-    const int per = (A.ntiles + NXCD - 1) / NXCD;
-    const int T = (int)(blockIdx.x % NXCD) * per + (int)(blockIdx.x / NXCD);
-    if (T >= A.ntiles) return;
-    int tx, ty, zc, b;
-    if constexpr (GEN) {
-        const int4 t = A.tiles[T];
-        tx = t.x; ty = t.y; zc = t.z; b = t.w;
-    } else {
-        int r = T;
-        ty = A.ty0 + r % A.nyi; r /= A.nyi;
-        tx = A.tx0 + r % A.nxi; r /= A.nxi;
-        zc = A.zcl[r % A.nzc];
-        b = r / A.nzc;
-    }
     const int lane = threadIdx.x, wv = threadIdx.y;
     const int tid = wv * TXH + lane;
     const int i0 = 1 + tx * TXE, j0 = 1 + ty * TYE;
@@ -508,6 +491,66 @@ __global__ __launch_bounds__(TXH * NW) void yee_fused(YeeArgs<R> A) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) row_nx[c] = row_pf[c];
         col_nx = col_pf;
+    }
+}
+
+template <typename S, typename R, int NW, int RPW, bool GEN, bool ZS>
+__global__ __launch_bounds__(TXH * NW) void yee_fused(YeeArgs<R> A) {
+    // GEN: general tiles. ZS (interior tiles of the first and last z chunks): x/y interior, z CPML
+    // slab applied per plane, no wall ghosts (z walls are CPML or PEC).
+    __shared__ R sE[3][NW * RPW + 1][TXH + 1];
+    __shared__ R sH[3][NW * RPW][TXH];
+    // ---- XCD-aware tile: XCD x runs the contiguous tile range [x*per, (x+1)*per)
+    const int per = (A.ntiles + NXCD - 1) / NXCD;
+    const int T = (int)(blockIdx.x % NXCD) * per + (int)(blockIdx.x / NXCD);
+    if (T >= A.ntiles) return;
+    int tx, ty, zc, b;
+    if constexpr (GEN) {
+        const int4 t = A.tiles[T];
+        tx = t.x; ty = t.y; zc = t.z; b = t.w;
+    } else {
+        int r = T;
+        ty = A.ty0 + r % A.nyi; r /= A.nyi;
+        tx = A.tx0 + r % A.nxi; r /= A.nxi;
+        zc = A.zcl[r % A.nzc];
+        b = r / A.nzc;
+    }
+    yee_tile<S, R, NW, RPW, GEN, ZS>(A, tx, ty, zc, b, sE, sH);
+}
+
+// Persistent step over the tiles of many problems (uneven grids) in one launch. list holds
+// (tx | kind << 16, ty, chunk, b | problem << 8) with kind 0 interior, 1 z-slab, 2 general;
+// the entries of XCD x are list[xs[x] .. xs[x+1]) and its workgroups (blockIdx % 8 == x) take
+// them in order from the counter ctr[parity * 8 + x]. The launch of one parity clears the
+// other parity's counters for the next step. Every tile runs the same yee_tile code as the
+// single-problem kernels.
+template <typename S, typename R, int NW, int RPW>
+__global__ __launch_bounds__(TXH * NW) void yee_batch(const YeeArgs<R>* __restrict__ args,
+                                                      const int4* __restrict__ list, const int* __restrict__ xs,
+                                                      unsigned* ctr, int parity) {
+    __shared__ R sE[3][NW * RPW + 1][TXH + 1];
+    __shared__ R sH[3][NW * RPW][TXH];
+    __shared__ int ticket;
+    const int x = (int)(blockIdx.x % NXCD);
+    const bool lead = threadIdx.x == 0 && threadIdx.y == 0;
+    if (blockIdx.x == 0 && threadIdx.y == 0 && threadIdx.x < NXCD)
+        atomicExch(&ctr[(parity ^ 1) * NXCD + threadIdx.x], 0u);
+    const int beg = xs[x], len = xs[x + 1] - beg;
+    for (;;) {
+        if (lead) ticket = (int)atomicAdd(&ctr[parity * NXCD + x], 1u);
+        __syncthreads();
+        const int t = ticket;
+        __syncthreads();  // every thread has read the ticket; LDS of the last tile is free
+        if (t >= len) break;
+        const int4 e = list[beg + t];
+        const int kind = e.x >> 16, tx = e.x & 0xffff, ty = e.y, zc = e.z, b = e.w & 0xff;
+        const YeeArgs<R>& A = args[e.w >> 8];
+        if (kind == 0)
+            yee_tile<S, R, NW, RPW, false, false>(A, tx, ty, zc, b, sE, sH);
+        else if (kind == 1)
+            yee_tile<S, R, NW, RPW, false, true>(A, tx, ty, zc, b, sE, sH);
+        else
+            yee_tile<S, R, NW, RPW, true, false>(A, tx, ty, zc, b, sE, sH);
     }
 }
 

@@ -77,6 +77,15 @@ def _declare(lib: C.CDLL) -> None:
         "oafd_debug_inject_fault": (C.c_int, [E, C.c_int]),
         "oafd_last_error": (C.c_char_p, [E]),
         "oafd_destroy": (None, [E]),
+        "oafd_batch_create": (C.c_int, [C.POINTER(C.POINTER(OafdDesc)), C.c_int, C.c_int, C.c_int64,
+                                        C.c_int, C.POINTER(E)]),
+        "oafd_batch_step": (C.c_int, [E, C.c_int64]),
+        "oafd_batch_step_timed": (C.c_int, [E, C.c_int64, P]),
+        "oafd_batch_count": (C.c_int, [E]),
+        "oafd_batch_step_count": (C.c_int64, [E]),
+        "oafd_batch_member": (E, [E, C.c_int]),
+        "oafd_batch_describe": (C.c_int, [E, C.c_char_p, C.c_int]),
+        "oafd_batch_destroy": (None, [E]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -90,6 +99,8 @@ EXPORTS = (
     "oafd_set_field", "oafd_step_count", "oafd_peak_bytes", "oafd_estimate_bytes",
     "oafd_step_timed", "oafd_describe", "oafd_plan", "oafd_copy_bandwidth",
     "oafd_debug_inject_fault", "oafd_last_error", "oafd_destroy",
+    "oafd_batch_create", "oafd_batch_step", "oafd_batch_step_timed", "oafd_batch_count",
+    "oafd_batch_step_count", "oafd_batch_member", "oafd_batch_describe", "oafd_batch_destroy",
 )
 
 

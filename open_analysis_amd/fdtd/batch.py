@@ -1,0 +1,141 @@
+"""Many compiled problems stepped together on one GPU (``oafd_batch_*`` in include/oafdtd.h).
+
+The problems may have different grids, materials, ports and probes. Each time step is one
+persistent fused-update launch over the tiles of every problem, plus one launch each of the
+face, source and probe kernels; the tile shape is picked by an autotuner over the whole batch.
+Member ``i`` behaves like a :class:`HipEngine` of problem ``i`` for read-out (``results``,
+``energy``, ``get_fields``, ``peak_bytes``); stepping goes through the batch.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import json
+import time
+from typing import Any
+
+import numpy as np
+
+from . import _lib
+from .engine import HipEngine, build_desc, estimate_bytes
+from .model import Compiled, Model, compile_model
+from .run import RunResult, source_spectrum
+
+
+class BatchMember(HipEngine):
+    """Read-out view of one problem of a :class:`HipBatch` (a borrowed engine handle)."""
+
+    def __init__(self, batch: HipBatch, index: int, c: Compiled, kinds: dict[str, Any]) -> None:
+        # no HipEngine.__init__: the handle belongs to the batch
+        self._batch = batch  # keeps the batch (and the handle) alive
+        self.c = c
+        self.dtype = batch.dtype
+        self.record_series = False
+        self.max_steps = batch.max_steps
+        self.use_graph = batch.use_graph
+        self.B = c.batch
+        self._probes = kinds
+        self._L = batch._L
+        self._h = C.c_void_p(self._L.oafd_batch_member(batch._h, index))
+
+    def __del__(self) -> None:  # the batch destroys its members
+        pass
+
+    def step(self, n: int = 1) -> None:
+        raise ValueError("a batch member is stepped through its batch (HipBatch.step)")
+
+    def step_timed(self, n: int) -> dict[str, float]:
+        raise ValueError("a batch member is stepped through its batch (HipBatch.step_timed)")
+
+
+class HipBatch:
+    """Problems ``cs`` stepped in lockstep in one persistent launch per step."""
+
+    def __init__(self, cs: list[Compiled], dtype: str = "float32", max_steps: int = 200_000,
+                 use_graph: bool = True) -> None:
+        if dtype not in _lib.DTYPES:
+            raise ValueError(f"dtype {dtype!r} is not one of {sorted(_lib.DTYPES)}")
+        if not cs:
+            raise ValueError("a batch needs at least one problem")
+        self.dtype = dtype
+        self.max_steps = int(max_steps)
+        self.use_graph = use_graph
+        self._L = _lib.lib()
+        descs, keeps, kinds = [], [], []
+        for c in cs:
+            d, keep, k = build_desc(c, self.max_steps)
+            descs.append(d)
+            keeps.append(keep)
+            kinds.append(k)
+        arr = (C.POINTER(_lib.OafdDesc) * len(descs))(*[C.pointer(d) for d in descs])
+        h = C.c_void_p()
+        _lib.check(self._L.oafd_batch_create(arr, len(descs), _lib.DTYPES[dtype], self.max_steps,
+                                             int(use_graph), C.byref(h)), "oafd_batch_create")
+        del keeps
+        self._h = h
+        self.members = [BatchMember(self, i, c, kinds[i]) for i, c in enumerate(cs)]
+
+    def __del__(self) -> None:
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._L.oafd_batch_destroy(h)
+            self._h = None
+
+    def __len__(self) -> int:
+        return len(self.members)
+
+    @property
+    def step_count(self) -> int:
+        return int(self._L.oafd_batch_step_count(self._h))
+
+    def step(self, n: int = 1) -> None:
+        _lib.check(self._L.oafd_batch_step(self._h, int(n)), "oafd_batch_step")
+
+    def step_timed(self, n: int) -> dict[str, float]:
+        out = np.zeros(4)
+        _lib.check(self._L.oafd_batch_step_timed(self._h, int(n), out.ctypes.data),
+                   "oafd_batch_step_timed")
+        return {"yee_ms": float(out[0]), "wall_ms": float(out[1]), "launches": float(out[2]),
+                "gap_ms": float(out[3])}
+
+    def describe(self) -> dict[str, Any]:
+        buf = C.create_string_buffer(1 << 16)
+        _lib.check(self._L.oafd_batch_describe(self._h, buf, len(buf)), "oafd_batch_describe")
+        return json.loads(buf.value.decode())
+
+
+def estimate_batch_bytes(cs: list[Compiled], dtype: str = "float32") -> int:
+    return sum(estimate_bytes(c, dtype) for c in cs)
+
+
+def run_batch(models: list[Model], steps: int, *, dtype: str = "float32",
+              compiled: list[Compiled] | None = None, use_graph: bool = True
+              ) -> tuple[list[RunResult], dict[str, Any]]:
+    """Run every model for ``steps`` steps in one batch (the fixed-length form of ``run``,
+    ``end_db=None``); returns one :class:`RunResult` per model and a timing report."""
+    t0 = time.perf_counter()
+    cs = compiled if compiled is not None else [compile_model(m, dtype) for m in models]
+    t1 = time.perf_counter()
+    batch = HipBatch(cs, dtype, max_steps=steps, use_graph=use_graph)
+    t2 = time.perf_counter()
+    batch.step(steps)
+    t3 = time.perf_counter()
+    out = []
+    for m, c, eng in zip(models, cs, batch.members, strict=True):
+        probes = eng.results()
+        sums = {f"V:{q['name']}": q["probe_parts"] for q in c.ports if q.get("probe_parts")}
+        sums.update(c.probe_sums)
+        for name, parts in sums.items():
+            probes[name] = sum(probes.pop(q) for q in parts)
+        out.append(RunResult(
+            probes=probes, source_dft=source_spectrum(c, steps), steps=steps, dt=c.dt,
+            engine="hip-batch", dtype=dtype, stopped_by="max_steps", energy_db=0.0,
+            peak_bytes=eng.peak_bytes(), report={"backend": "rocm"},
+            port_r=np.array([float(q["r_ohm"]) for q in c.ports])))
+    cells = sum(c.grid.cells * c.batch for c in cs)
+    report = {"compile_s": t1 - t0, "setup_s": t2 - t1, "step_s": t3 - t2,
+              "cell_updates": float(cells) * steps,
+              "rate_mcells": float(cells) * steps / max(t3 - t2, 1e-9) / 1e6,
+              "tiling": batch.describe()}
+    del batch
+    return out, report

@@ -135,3 +135,21 @@ def test_extended_table_keeps_coefficients() -> None:
     for ci in range(3):
         keep = t["idx"][ci] < n0
         np.testing.assert_array_equal(t["idx"][ci][keep], c.idx[ci][keep])
+
+
+@needs_lib
+def test_batch_needs_a_device_and_valid_arguments() -> None:
+    """Batch creation checks its arguments, then refuses to run without a device (CPU)."""
+    from open_analysis_amd.fdtd import vias
+
+    L = _lib.lib()
+    h = C.c_void_p()
+    assert L.oafd_batch_create(None, 0, 0, 10, 1, C.byref(h)) == _lib.OAFD_E_INVALID
+    models, _ = vias.geometry_set(2, seed=3, small=True)
+    descs = [build_desc(compile_model(m), 10) for m in models]
+    arr = (C.POINTER(_lib.OafdDesc) * 2)(*[C.pointer(d[0]) for d in descs])
+    assert L.oafd_batch_create(arr, 2, 7, 10, 1, C.byref(h)) == _lib.OAFD_E_INVALID
+    rc = L.oafd_batch_create(arr, 2, 0, 10, 1, C.byref(h))
+    if L.oafd_device_count() == 0:
+        assert rc == _lib.OAFD_E_NO_DEVICE and not h.value
+    assert L.oafd_batch_count(None) == 0 and not L.oafd_batch_member(None, 0)

@@ -1,0 +1,125 @@
+"""Many uneven problems in one persistent launch per step (GPU).
+
+Every member of a batch must give exactly the bits it gives when run alone (the same tile code
+and arithmetic order), whatever else is in the batch; and the via geometries must agree with the
+CPU oracle.
+"""
+
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+from open_analysis_amd.fdtd import compile_model, sparams, vias
+from open_analysis_amd.fdtd.batch import HipBatch, run_batch
+from open_analysis_amd.fdtd.engine import HipEngine
+from oracle import OracleEngine
+
+pytestmark = pytest.mark.gpu
+STEPS = 300
+
+
+def _say(msg: str) -> None:
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def small_set():
+    models, specs = vias.geometry_set(4, seed=5, freqs=np.linspace(2e9, 20e9, 10), small=True)
+    cs = [compile_model(m) for m in models]
+    _say("[batch] grids " + ", ".join(f"{s.nx}x{s.ny}x{s.nz}" for s in specs))
+    return models, cs
+
+
+def _state(eng) -> tuple[dict, dict]:
+    return eng.get_fields(), eng.results()
+
+
+def _assert_same(a: tuple[dict, dict], b: tuple[dict, dict]) -> None:
+    for k in a[0]:
+        assert np.array_equal(a[0][k], b[0][k]), k
+    for k in a[1]:
+        assert np.array_equal(a[1][k], b[1][k]), k
+
+
+def test_batch_members_match_single_runs_bitwise(small_set) -> None:
+    _, cs = small_set
+    batch = HipBatch(cs, "float32", max_steps=STEPS)
+    batch.step(STEPS)
+    for i, c in enumerate(cs):
+        _say(f"[batch] member {i} vs its own engine")
+        eng = HipEngine(c, "float32", max_steps=STEPS)
+        eng.step(STEPS)
+        _assert_same(_state(batch.members[i]), _state(eng))
+        assert batch.members[i].energy() == eng.energy()
+
+
+def test_subset_batch_is_invariant(small_set) -> None:
+    _, cs = small_set
+    full = HipBatch(cs, "float32", max_steps=STEPS)
+    full.step(STEPS)
+    sub = HipBatch([cs[2], cs[0]], "float32", max_steps=STEPS)
+    sub.step(STEPS)
+    _assert_same(_state(full.members[2]), _state(sub.members[0]))
+    _assert_same(_state(full.members[0]), _state(sub.members[1]))
+
+
+def test_batch_graph_matches_eager(small_set) -> None:
+    _, cs = small_set
+    a = HipBatch(cs[:2], "float32", max_steps=200, use_graph=True)
+    b = HipBatch(cs[:2], "float32", max_steps=200, use_graph=False)
+    for n in (64, 64, 7, 57, 8):  # crosses graph chunks and eager remainders
+        a.step(n)
+        b.step(n)
+    assert a.step_count == b.step_count == 200
+    for i in range(2):
+        _assert_same(_state(a.members[i]), _state(b.members[i]))
+
+
+def test_batch_member_matches_oracle(small_set) -> None:
+    _, cs = small_set
+    c = cs[1]
+    batch = HipBatch(cs, "float32", max_steps=STEPS)
+    batch.step(STEPS)
+    ref = OracleEngine(c, "float64")
+    ref.step(STEPS)
+    got = batch.members[1]
+    fa, fb = got.get_fields(), ref.get_fields()
+    for kind in "EH":
+        scale = max(np.abs(fb[kind + x]).max() for x in "xyz")
+        for x in "xyz":
+            assert np.abs(fa[kind + x] - fb[kind + x]).max() <= 1e-4 * scale, kind + x
+    ra, rb = got.results(), ref.results()
+    for k in rb:
+        assert np.abs(ra[k] - rb[k]).max() <= 1e-4 * np.abs(rb[k]).max(), k
+
+
+def test_member_refuses_its_own_step(small_set) -> None:
+    _, cs = small_set
+    batch = HipBatch(cs[:1], "float32", max_steps=10)
+    with pytest.raises(ValueError):
+        batch.members[0].step(1)
+    with pytest.raises(ValueError):
+        batch.step(11)
+
+
+def test_run_batch_sparams(small_set) -> None:
+    """S-parameters of a batch run equal those of the members run alone."""
+    models, cs = small_set
+    res, rep = run_batch(models[:2], 600, compiled=cs[:2])
+    _say(f"[batch] run_batch {rep['rate_mcells']:.0f} Mcell/s, tiling {rep['tiling']['waves']}x"
+         f"{rep['tiling']['rows_per_wave']}")
+    for i in range(2):
+        eng = HipEngine(cs[i], "float32", max_steps=600)
+        eng.step(600)
+        s_batch = sparams(models[i], res[i])
+        single = eng.results()
+        for q in cs[i].ports:
+            key = f"V:{q['name']}"
+            parts = q.get("probe_parts")
+            v = sum(single[p] for p in parts) if parts else single[key]
+            assert np.array_equal(res[i].probes[key], v), key
+        assert np.all(np.isfinite(s_batch))
