@@ -125,7 +125,7 @@ def bench_batch(a) -> dict:
     t_compile = time.perf_counter() - t0
     print(f"[bench] compiled {len(cs)} geometries in {t_compile:.1f} s", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
-    batch = HipBatch(cs, a.dtype, max_steps=a.warmup + a.steps + 8)
+    batch = HipBatch(cs, a.dtype, max_steps=a.warmup + a.steps + 64)
     t_setup = time.perf_counter() - t0
     batch.step(a.warmup)
     print(f"[bench] batch ready ({t_setup:.1f} s), warm-up done", file=sys.stderr, flush=True)

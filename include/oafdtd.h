@@ -126,6 +126,7 @@ int oafd_step(oafd_engine* e, int64_t n);
 int oafd_energy(oafd_engine* e, double* out_joules);
 /* out: (n_freq, batch, n_probe[kind], 2) float64 (re, im) */
 int oafd_results(oafd_engine* e, int kind, double* out);
+/* steps recorded in the probe series */
 int64_t oafd_series_len(const oafd_engine* e);
 /* out: (steps, batch, n_probe[kind]) float64; cap = capacity in elements */
 int oafd_series(oafd_engine* e, int kind, double* out, int64_t cap);
@@ -140,7 +141,8 @@ int64_t oafd_estimate_bytes(const oafd_desc* d, int dtype);
  * out[2] = fused launches, out[3] = summed ms of the face/source/probe kernels. */
 int oafd_step_timed(oafd_engine* e, int64_t n, double* out);
 /* JSON text: the fused-step tile shape the autotuner chose and the timing of every candidate.
- * OAFD_YEE_VARIANT=<n> in the environment forces candidate n (tests run each one). */
+ * OAFD_YEE_VARIANT=<n> or <n>,<x> in the environment forces tile shape n (tests run each one)
+ * and, with x = 1, the x-fastest tile launch order. */
 int oafd_describe(const oafd_engine* e, char* buf, int len);
 /* Host only (no device): the fused step's tile plan for a grid. npml = slab widths
  * (xlo xhi ylo yhi zlo zhi); zslab = 1 when no z wall is PMC; kb = z chunk boundaries
@@ -160,7 +162,7 @@ int oafd_copy_bandwidth(int64_t bytes, int reps, double* out);
  * batch (OAFD_YEE_VARIANT forces one). Each problem may have at most 255 members. Member i is a
  * borrowed oafd_engine handle: oafd_energy / results / get_field / set_field / peak_bytes /
  * step_count work on it, oafd_step on it returns OAFD_E_INVALID and oafd_destroy ignores it;
- * it lives until oafd_batch_destroy. Series recording is not available in a batch. */
+ * it lives until oafd_batch_destroy. */
 typedef struct oafd_batch oafd_batch;
 int oafd_batch_create(const oafd_desc* const* descs, int n, int dtype, int64_t max_steps, int use_graph,
                       oafd_batch** out);
@@ -168,6 +170,9 @@ int oafd_batch_step(oafd_batch* b, int64_t n);
 /* eager steps with HIP events: out[0] = summed persistent-launch ms, out[1] = wall ms,
  * out[2] = launches, out[3] = summed ms between launches */
 int oafd_batch_step_timed(oafd_batch* b, int64_t n, double* out);
+/* Record every member's probe series (oafd_series on the member handle) for the first `steps`
+ * steps, rounded up to whole 64-step chunks; call before the first step. */
+int oafd_batch_record(oafd_batch* b, int64_t steps);
 int oafd_batch_count(const oafd_batch* b);
 int64_t oafd_batch_step_count(const oafd_batch* b);
 oafd_engine* oafd_batch_member(oafd_batch* b, int i);

@@ -2,8 +2,9 @@
 
     python -m open_analysis_amd.csrc.build
 
-Compiles ``oafdtd.hip`` with hipcc into ``open_analysis_amd/csrc/liboafdtd_hip.so`` (in-tree, so
-it travels with the source tree). Packed FP32 instructions are disabled
+Compiles ``oafdtd.hip`` into ``open_analysis_amd/csrc/liboafdtd_hip.so`` (the engine) and
+``ringdown.hip`` into ``liboaring_hip.so`` (the learned ringdown extrapolator) with hipcc,
+in-tree, so they travel with the source tree. Packed FP32 instructions are disabled
 (``-target-feature -packed-fp32-ops``); the compiler still vectorises ordinary float code.
 """
 
@@ -18,8 +19,8 @@ from pathlib import Path
 HERE = Path(__file__).resolve().parent
 ROOT = HERE.parents[1]
 LIB = HERE / "liboafdtd_hip.so"
-SOURCES = [HERE / "oafdtd.hip"]
-DEPS = SOURCES + [HERE / "kernels.hpp", ROOT / "include" / "oafdtd.h"]
+RING_LIB = HERE / "liboaring_hip.so"
+TARGETS = {LIB: HERE / "oafdtd.hip", RING_LIB: HERE / "ringdown.hip"}
 
 
 def hipcc() -> str:
@@ -29,23 +30,36 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found (ROCm is required to build the engine)")
 
 
-def build(verbose: bool = False, extra: list[str] | None = None) -> Path:
-    """Compile the library (always; the call is the build check)."""
-    tmp = LIB.with_suffix(".so.tmp")
+def _command(src: Path, out: Path, extra: list[str] | None) -> list[str]:
     cmd = [
         hipcc(), "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-        "-munsafe-fp-atomics", "-I", str(ROOT / "include"),
-        *[str(s) for s in SOURCES], "-o", str(tmp),
+        "-munsafe-fp-atomics", "-I", str(ROOT / "include"), str(src), "-o", str(out),
     ]
     if extra:
         cmd[1:1] = extra
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    if verbose or res.returncode != 0:
-        sys.stderr.write(res.stdout + res.stderr)
-    if res.returncode != 0:
-        raise RuntimeError(f"hipcc failed ({res.returncode}): {' '.join(cmd)}")
-    os.replace(tmp, LIB)
+    return cmd
+
+
+def build(verbose: bool = False, extra: list[str] | None = None) -> Path:
+    """Compile both libraries (always; the call is the build check), in parallel."""
+    procs = []
+    for lib, src in TARGETS.items():
+        tmp = lib.with_suffix(".so.tmp")
+        cmd = _command(src, tmp, extra)
+        procs.append((lib, tmp, cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE,
+                                                      stderr=subprocess.PIPE, text=True)))
+    failed = []
+    for lib, tmp, cmd, p in procs:
+        out, err = p.communicate()
+        if verbose or p.returncode != 0:
+            sys.stderr.write(out + err)
+        if p.returncode != 0:
+            failed.append(f"hipcc failed ({p.returncode}): {' '.join(cmd)}")
+        else:
+            os.replace(tmp, lib)
+    if failed:
+        raise RuntimeError("\n".join(failed))
     return LIB
 
 

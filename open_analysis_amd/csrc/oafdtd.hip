@@ -70,6 +70,7 @@ struct oafd_engine {
     virtual int set_field(int comp, const double* in) = 0;
     virtual int step_timed(int64_t n, double* out) = 0;
     virtual std::string describe() const = 0;
+    virtual int64_t series_len() const { return step_count; }
     int64_t step_count = 0;
     int64_t max_steps = 0;
     int64_t peak = 0;
@@ -122,6 +123,7 @@ struct YeePlan {
     int nzs = 0;
     int n_chunks = 0;
     int nck = 0;      // requested interior chunks (0 = default)
+    int xfast = 0;    // launch order of tiles: x fastest (1) or y fastest (0)
     double ms = 0.0;  // autotuner timing of one step
 };
 
@@ -138,7 +140,7 @@ struct HostPlan {
 
 // nck_req: number of interior z chunks (0 = one chunk per 8192 / tiles-per-plane workgroups).
 HostPlan plan_tiles(int nx, int ny, int nz, int B, const int npml[3][2], int nw, int rpw, bool zslab_ok,
-                    int nck_req = 0) {
+                    int nck_req = 0, bool xfast = false) {
     HostPlan P;
     const int tye = nw * rpw - 1;
     const int ntx = (nx + 1 + oafd::TXE - 1) / oafd::TXE;
@@ -171,13 +173,14 @@ HostPlan plan_tiles(int nx, int ny, int nz, int B, const int npml[3][2], int nw,
     if (nzc > 0 && zslab_ok) P.zs = {0, nch - 1};  // the slab chunks around the interior range
     for (int bb = 0; bb < B; ++bb)
         for (int zc = 0; zc < nch; ++zc)
-            for (int tx = 0; tx < ntx; ++tx)
-                for (int ty = 0; ty < nty; ++ty) {
-                    const bool xy_in = tx >= tx_lo && tx <= tx_hi && ty >= ty_lo && ty <= ty_hi;
-                    const bool inner = nzc > 0 && zc >= zc0 && zc < zc0 + nzc && xy_in;
-                    const bool zs = nzc > 0 && zslab_ok && (zc == 0 || zc == nch - 1) && xy_in;
-                    if (!inner && !zs) P.gen.push_back(make_int4(tx, ty, zc, bb));
-                }
+            for (int q = 0; q < ntx * nty; ++q) {
+                const int tx = xfast ? q % ntx : q / nty;
+                const int ty = xfast ? q / ntx : q % nty;
+                const bool xy_in = tx >= tx_lo && tx <= tx_hi && ty >= ty_lo && ty <= ty_hi;
+                const bool inner = nzc > 0 && zc >= zc0 && zc < zc0 + nzc && xy_in;
+                const bool zs = nzc > 0 && zslab_ok && (zc == 0 || zc == nch - 1) && xy_in;
+                if (!inner && !zs) P.gen.push_back(make_int4(tx, ty, zc, bb));
+            }
     if (nzc > 0) {
         P.tx0 = tx_lo;
         P.ty0 = ty_lo;
@@ -503,13 +506,14 @@ struct Engine final : oafd_engine {
     }
 
     // ---- fused-step planning: z chunks aligned with the CPML slabs, interior box, general list
-    YeePlan make_plan(int v, int nck = 0) {
+    YeePlan make_plan(int v, int nck = 0, int xfast = 0) {
         YeePlan P;
         P.variant = v;
         P.nck = nck;
+        P.xfast = xfast;
         P.nw = vars[v].nw;
         P.rpw = vars[v].rpw;
-        const HostPlan H = plan_tiles(nx, ny, nz, B, npml_, P.nw, P.rpw, zslab_ok(), nck);
+        const HostPlan H = plan_tiles(nx, ny, nz, B, npml_, P.nw, P.rpw, zslab_ok(), nck, xfast != 0);
         P.n_chunks = (int)H.kb.size() - 1;
         P.kb = dev_vec(H.kb);
         P.n_gen = (int)H.gen.size();
@@ -548,6 +552,7 @@ struct Engine final : oafd_engine {
         A.ty0 = plan.ty0;
         A.nxi = plan.nxi;
         A.nyi = plan.nyi;
+        A.xfast = plan.xfast;
         return A;
     }
     bool zslab_ok() const { return !base.pmc_lo[2] && !base.pmc_hi[2]; }
@@ -579,9 +584,11 @@ struct Engine final : oafd_engine {
         vars = variants<S, R>();
         const char* force = std::getenv("OAFD_YEE_VARIANT");
         const long long cells = (long long)bs * B;
-        if (force && *force) {
-            const int v = std::min<int>(std::max(0, std::atoi(force)), (int)vars.size() - 1);
-            plan = make_plan(v);
+        if (force && *force) {  // "<shape>" or "<shape>,<x fastest 0|1>"
+            char* rest = nullptr;
+            const long v = std::strtol(force, &rest, 10);
+            const int xf = (rest && *rest == ',') ? (std::atoi(rest + 1) != 0) : 0;
+            plan = make_plan(std::min<int>(std::max<int>(0, (int)v), (int)vars.size() - 1), 0, xf);
             return;
         }
         if (cells < (1LL << 22)) {
@@ -592,23 +599,33 @@ struct Engine final : oafd_engine {
         HIP_TRY(hipEventCreate(&e0));
         HIP_TRY(hipEventCreate(&e1));
         int best = -1;
-        static const int chunk_counts[] = {0, 1, 2, 3, 4, 6, 8, 12};
-        for (int v = 0; v < (int)vars.size(); ++v) {
-            for (int nck : chunk_counts) {
-                plan = make_plan(v, nck);
-                if (nck > 0 && plan.nzc != nck) continue;  // not that many planes: same as a smaller count
-                for (int s = 0; s < 2; ++s) launch_yee(s & 1);
-                HIP_TRY(hipEventRecord(e0, stream));
-                for (int s = 0; s < 4; ++s) launch_yee(s & 1);
-                HIP_TRY(hipEventRecord(e1, stream));
-                HIP_TRY(hipEventSynchronize(e1));
-                float ms = 0.f;
-                HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-                plan.ms = ms / 4.0;
-                tried.push_back(plan);
-                if (best < 0 || plan.ms < tried[best].ms) best = (int)tried.size() - 1;
-            }
-        }
+        auto trial = [&](int v, int nck, int xf) {
+            plan = make_plan(v, nck, xf);
+            if (nck > 0 && plan.nzc != nck) return;  // not that many planes: same as a smaller count
+            for (int s = 0; s < 2; ++s) launch_yee(s & 1);
+            HIP_TRY(hipEventRecord(e0, stream));
+            for (int s = 0; s < 4; ++s) launch_yee(s & 1);
+            HIP_TRY(hipEventRecord(e1, stream));
+            HIP_TRY(hipEventSynchronize(e1));
+            float ms = 0.f;
+            HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+            plan.ms = ms / 4.0;
+            tried.push_back(plan);
+            if (best < 0 || plan.ms < tried[best].ms) best = (int)tried.size() - 1;
+        };
+        // every tile shape in both launch orders at the default chunking, then the chunk
+        // counts for the best two (shape, order) pairs
+        for (int v = 0; v < (int)vars.size(); ++v)
+            for (int xf = 0; xf < 2; ++xf) trial(v, 0, xf);
+        std::vector<int> order((int)tried.size());
+        for (int q = 0; q < (int)order.size(); ++q) order[q] = q;
+        std::sort(order.begin(), order.end(), [&](int a, int b) { return tried[a].ms < tried[b].ms; });
+        std::vector<std::pair<int, int>> finalists;
+        for (int q = 0; q < (int)order.size() && (int)finalists.size() < 2; ++q)
+            finalists.emplace_back(tried[order[q]].variant, tried[order[q]].xfast);
+        static const int chunk_counts[] = {1, 2, 3, 4, 6, 8, 12, 16};
+        for (auto& f : finalists)
+            for (int nck : chunk_counts) trial(f.first, nck, f.second);
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         plan = tried[best];
@@ -638,14 +655,16 @@ struct Engine final : oafd_engine {
         std::snprintf(fbuf, sizeof fbuf, "{\"field_scale\": %.6g, ", fs);
         std::string s = std::string(fbuf) + "\"variant\": ";
         std::snprintf(buf, sizeof buf, "{\"waves\": %d, \"rows_per_wave\": %d, \"tile_e\": [%d, %d], "
-                      "\"general_tiles\": %d, \"interior_grid\": [%u, %u, %u], \"chunks\": %d}",
+                      "\"general_tiles\": %d, \"interior_grid\": [%u, %u, %u], \"chunks\": %d, "
+                      "\"x_fastest\": %d}",
                       plan.nw, plan.rpw, oafd::TXE, plan.nw * plan.rpw - 1, plan.n_gen, (unsigned)plan.nxi,
-                      (unsigned)plan.nyi, (unsigned)(plan.nzc * B), plan.n_chunks);
+                      (unsigned)plan.nyi, (unsigned)(plan.nzc * B), plan.n_chunks, plan.xfast);
         s += buf;
         s += ", \"tuning_ms\": [";
         for (size_t q = 0; q < tried.size(); ++q) {
-            std::snprintf(buf, sizeof buf, "%s{\"waves\": %d, \"rows_per_wave\": %d, \"chunks\": %d, \"ms\": %.5f}",
-                          q ? ", " : "", tried[q].nw, tried[q].rpw, tried[q].nzc, tried[q].ms);
+            std::snprintf(buf, sizeof buf, "%s{\"waves\": %d, \"rows_per_wave\": %d, \"chunks\": %d, "
+                          "\"x_fastest\": %d, \"ms\": %.5f}",
+                          q ? ", " : "", tried[q].nw, tried[q].rpw, tried[q].nzc, tried[q].xfast, tried[q].ms);
             s += buf;
         }
         s += "]}";
@@ -746,6 +765,7 @@ struct Engine final : oafd_engine {
         graph = g;
         graph_state = 1;
     }
+    int64_t series_len() const override { return series_steps; }
     void pull_series(int slot0, int count) {
         if (!record_series) return;
         HIP_TRY(hipStreamSynchronize(stream));
@@ -919,6 +939,7 @@ struct BatchBase {
     virtual int step_timed(int64_t n, double* out) = 0;
     virtual std::string describe() const = 0;
     virtual oafd_engine* member(int i) = 0;
+    virtual void record(int64_t steps) = 0;
     int count = 0;
     int64_t step_count = 0;
     int64_t max_steps = 0;
@@ -930,7 +951,7 @@ struct Batch final : BatchBase {
     hipStream_t stream = nullptr;
     std::vector<void*> allocs;
     std::vector<YeeVariant<R>> vars;
-    int variant = 0, nck = 0, grid = 0, nlist = 0;
+    int variant = 0, nck = 0, xfast = 0, grid = 0, nlist = 0;
     oafd::YeeArgs<R>* dargs[2] = {nullptr, nullptr};
     int4* list = nullptr;
     int* xs = nullptr;
@@ -945,7 +966,7 @@ struct Batch final : BatchBase {
     hipGraphExec_t graph_exec = nullptr;
     int graph_state = 0;
     struct Trial {
-        int variant, nck, tiles;
+        int variant, nck, xfast, tiles;
         double ms;
     };
     std::vector<Trial> tried;
@@ -974,6 +995,15 @@ struct Batch final : BatchBase {
     }
 
     oafd_engine* member(int i) override { return (i >= 0 && i < count) ? mem[i] : nullptr; }
+    int64_t record_until = 0;  // members record their probe series while step_count < this
+    void record(int64_t steps) override {
+        record_until = steps;
+        for (auto* m : mem) m->record_series = steps > 0;
+    }
+    void pull(int64_t start, int slot0, int take) {
+        if (start >= record_until) return;
+        for (auto* m : mem) m->pull_series(slot0, take);
+    }
 
     void init(const oafd_desc* const* descs, int n, int64_t maxs, bool graph_on) {
         count = n;
@@ -1021,15 +1051,16 @@ struct Batch final : BatchBase {
         tune(cells);
     }
 
-    // Tile list of every member for tile shape v and interior chunk count nck (0 = default).
-    void build(int v, int nck_req) {
+    // Tile list of every member for tile shape v, interior chunk count nck (0 = default) and
+    // launch order xf (1: x fastest).
+    void build(int v, int nck_req, int xf = 0) {
         const int nw = vars[v].nw, rpw = vars[v].rpw;
         std::vector<std::vector<int4>> part(oafd::NXCD);
         std::vector<oafd::YeeArgs<R>> args[2];
         for (int mi = 0; mi < count; ++mi) {
             Engine<S, R>& m = *mem[mi];
-            m.plan = m.make_plan(v, nck_req);
-            const HostPlan H = plan_tiles(m.nx, m.ny, m.nz, m.B, m.npml_, nw, rpw, m.zslab_ok(), nck_req);
+            m.plan = m.make_plan(v, nck_req, xf);
+            const HostPlan H = plan_tiles(m.nx, m.ny, m.nz, m.B, m.npml_, nw, rpw, m.zslab_ok(), nck_req, xf != 0);
             std::vector<int4> ent;
             auto add = [&](int kind, int tx, int ty, int zc, int b) {
                 ent.push_back(make_int4(tx | (kind << 16), ty, zc, b | (mi << 8)));
@@ -1043,13 +1074,14 @@ struct Batch final : BatchBase {
                         for (int ty = H.ty0; ty < H.ty0 + H.nyi; ++ty) add(1, tx, ty, zc, b);
             }
             for (const int4& g : H.gen) add(2, g.x, g.y, g.z, g.w);
-            // member order (b, chunk, tx, ty), ty fastest: neighbours in y share E halo rows
-            std::sort(ent.begin(), ent.end(), [](const int4& a, const int4& b) {
+            // member order (b, chunk, then y and x with the planned one fastest)
+            std::sort(ent.begin(), ent.end(), [xf](const int4& a, const int4& b) {
                 const int ba = a.w & 0xff, bb = b.w & 0xff;
                 if (ba != bb) return ba < bb;
                 if (a.z != b.z) return a.z < b.z;
-                if ((a.x & 0xffff) != (b.x & 0xffff)) return (a.x & 0xffff) < (b.x & 0xffff);
-                return a.y < b.y;
+                const int ax = a.x & 0xffff, bx = b.x & 0xffff;
+                if (xf) return a.y != b.y ? a.y < b.y : ax < bx;
+                return ax != bx ? ax < bx : a.y < b.y;
             });
             // one contiguous eighth of every member per XCD queue
             const size_t len = ent.size();
@@ -1071,6 +1103,7 @@ struct Batch final : BatchBase {
         for (int p = 0; p < 2; ++p) dargs[p] = dev<oafd::YeeArgs<R>>(args[p].size(), args[p].data());
         variant = v;
         nck = nck_req;
+        xfast = xf;
         // persistent grid: the resident workgroups of the device, a multiple of the XCD count
         int dev_id = 0, cus = 0, per_cu = 0;
         HIP_TRY(hipGetDevice(&dev_id));
@@ -1112,8 +1145,11 @@ struct Batch final : BatchBase {
     // Autotuner over tile shapes and interior chunk counts, timing the persistent launch alone.
     void tune(long long cells) {
         const char* force = std::getenv("OAFD_YEE_VARIANT");
-        if (force && *force) {
-            build(std::min<int>(std::max(0, std::atoi(force)), (int)vars.size() - 1), 0);
+        if (force && *force) {  // "<shape>" or "<shape>,<x fastest 0|1>"
+            char* rest = nullptr;
+            const long v = std::strtol(force, &rest, 10);
+            const int xf = (rest && *rest == ',') ? (std::atoi(rest + 1) != 0) : 0;
+            build(std::min<int>(std::max<int>(0, (int)v), (int)vars.size() - 1), 0, xf);
         } else if (cells < (1LL << 22)) {
             build(0, 0);
         } else {
@@ -1123,8 +1159,9 @@ struct Batch final : BatchBase {
             int best = -1;
             static const int chunk_counts[] = {0, 2, 4};
             for (int v = 0; v < (int)vars.size(); ++v)
-                for (int c : chunk_counts) {
-                    build(v, c);
+                for (int t = 0; t < 6; ++t) {
+                    const int c = chunk_counts[t % 3], xf = t / 3;
+                    build(v, c, xf);
                     reset_counters();
                     for (int s = 0; s < 2; ++s) launch_yee(s & 1);
                     HIP_TRY(hipEventRecord(e0, stream));
@@ -1133,12 +1170,12 @@ struct Batch final : BatchBase {
                     HIP_TRY(hipEventSynchronize(e1));
                     float ms = 0.f;
                     HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-                    tried.push_back({v, c, nlist, ms / 4.0});
+                    tried.push_back({v, c, xf, nlist, ms / 4.0});
                     if (best < 0 || tried.back().ms < tried[best].ms) best = (int)tried.size() - 1;
                 }
             (void)hipEventDestroy(e0);
             (void)hipEventDestroy(e1);
-            build(tried[best].variant, tried[best].nck);
+            build(tried[best].variant, tried[best].nck, tried[best].xfast);
             for (auto* m : mem) {  // the timed steps ran on zero fields; clear every state anyway
                 const size_t nfield = (size_t)m->B * m->bs * sizeof(S);
                 for (int p = 0; p < 2; ++p)
@@ -1217,6 +1254,7 @@ struct Batch final : BatchBase {
                     HIP_TRY(hipGetLastError());
                 }
             }
+            pull(start, in_chunk, take);
             advance_members(take);
             done += take;
         }
@@ -1251,6 +1289,7 @@ struct Batch final : BatchBase {
                 hipLaunchKernelGGL(oafd::advance_step, dim3(1), dim3(64), 0, stream, step_base, CHUNK);
                 HIP_TRY(hipGetLastError());
             }
+            pull(start, in_chunk, take);
             advance_members(take);
             done += take;
         }
@@ -1279,13 +1318,14 @@ struct Batch final : BatchBase {
         char buf[256];
         std::snprintf(buf, sizeof buf,
                       "{\"problems\": %d, \"waves\": %d, \"rows_per_wave\": %d, \"chunks_requested\": %d, "
-                      "\"tiles\": %d, \"grid\": %d, \"tuning_ms\": [",
-                      count, vars[variant].nw, vars[variant].rpw, nck, nlist, grid);
+                      "\"x_fastest\": %d, \"tiles\": %d, \"grid\": %d, \"tuning_ms\": [",
+                      count, vars[variant].nw, vars[variant].rpw, nck, xfast, nlist, grid);
         std::string s = buf;
         for (size_t q = 0; q < tried.size(); ++q) {
             std::snprintf(buf, sizeof buf, "%s{\"waves\": %d, \"rows_per_wave\": %d, \"chunks_requested\": %d, "
-                          "\"tiles\": %d, \"ms\": %.5f}", q ? ", " : "", vars[tried[q].variant].nw,
-                          vars[tried[q].variant].rpw, tried[q].nck, tried[q].tiles, tried[q].ms);
+                          "\"x_fastest\": %d, \"tiles\": %d, \"ms\": %.5f}", q ? ", " : "",
+                          vars[tried[q].variant].nw, vars[tried[q].variant].rpw, tried[q].nck, tried[q].xfast,
+                          tried[q].tiles, tried[q].ms);
             s += buf;
         }
         s += "]}";
@@ -1433,10 +1473,7 @@ int oafd_describe(const oafd_engine* e, char* buf, int len) {
     return (int)s.size() < len ? OAFD_OK : fail(OAFD_E_INVALID, "describe buffer too small");
 }
 
-int64_t oafd_series_len(const oafd_engine* e) {
-    if (!e) return 0;
-    return static_cast<const oafd_engine*>(e)->step_count;  // series length equals steps when recorded
-}
+int64_t oafd_series_len(const oafd_engine* e) { return e ? e->series_len() : 0; }
 int64_t oafd_step_count(const oafd_engine* e) { return e ? e->step_count : 0; }
 int64_t oafd_peak_bytes(const oafd_engine* e) { return e ? e->peak : 0; }
 int64_t oafd_estimate_bytes(const oafd_desc* d, int dtype) {
@@ -1599,6 +1636,13 @@ int oafd_batch_step(oafd_batch* b, int64_t n) { BATCH_CALL(b, return b->impl->st
 int oafd_batch_step_timed(oafd_batch* b, int64_t n, double* out) {
     if (!out) return fail(OAFD_E_INVALID, "null output");
     BATCH_CALL(b, return b->impl->step_timed(n, out));
+}
+int oafd_batch_record(oafd_batch* b, int64_t steps) {
+    if (!b || !b->impl) return fail(OAFD_E_INVALID, "null batch");
+    if (steps < 0) return fail(OAFD_E_INVALID, "negative step count");
+    if (b->impl->step_count > 0) return fail(OAFD_E_INVALID, "recording must be set before the first step");
+    b->impl->record(steps);
+    return OAFD_OK;
 }
 int oafd_batch_count(const oafd_batch* b) { return (b && b->impl) ? b->impl->count : 0; }
 int64_t oafd_batch_step_count(const oafd_batch* b) { return (b && b->impl) ? b->impl->step_count : 0; }

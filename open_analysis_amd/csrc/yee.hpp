@@ -19,8 +19,9 @@
 // touch no CPML slab and no wall and has no boundary code; the general kernel (GEN = true) adds
 // the CPML psi recursions (psi in slab arrays addressed by a 1-D slot map per axis, with the slot
 // and coefficient lookups hoisted out of the z loop), PMC mirror ghosts and the write masks of
-// the staggered lattice. Workgroups are mapped to tiles so that consecutive tiles of the y
-// direction (which share E halo rows) run on the same XCD.
+// the staggered lattice. Workgroups are mapped to tiles so that consecutive tiles (neighbours in
+// x, which share the cache lines their rows straddle, or in y, which share E halo rows; the
+// autotuner picks the order) run on the same XCD at the same time.
 //
 // Equations: packages/oa-fdtd/src/oa_fdtd/reference.py:152-266 (order :251-266) and
 // kernels/yee.cu:30-81, cpml :139-168; the Debye post-update (physics.cu:29-51) is folded into
@@ -54,6 +55,7 @@ struct YeeArgs {
     const int4* tiles;   // general kernel: (tx, ty, chunk, b), ty fastest
     int ntiles;          // tiles of this launch
     int tx0, ty0, nxi, nyi, nzc;  // interior kernels: tile box in x, y and a chunk count
+    int xfast;           // interior kernels: consecutive tiles step in x (else in y)
     const int* zcl;      // interior kernels: the chunks of this launch
     int pmc_lo[3], pmc_hi[3];
     int npml[3][2];      // CPML slab widths per axis and side (0 = none)
@@ -500,7 +502,8 @@ __global__ __launch_bounds__(TXH * NW) void yee_fused(YeeArgs<R> A) {
     // slab applied per plane, no wall ghosts (z walls are CPML or PEC).
     __shared__ R sE[3][NW * RPW + 1][TXH + 1];
     __shared__ R sH[3][NW * RPW][TXH];
-    // ---- XCD-aware tile: XCD x runs the contiguous tile range [x*per, (x+1)*per)
+    // ---- XCD-aware tile: XCD x runs the contiguous tile range [x*per, (x+1)*per), so tiles
+    // that are neighbours in the launch order (in x or in y, as planned) share halo lines in L2
     const int per = (A.ntiles + NXCD - 1) / NXCD;
     const int T = (int)(blockIdx.x % NXCD) * per + (int)(blockIdx.x / NXCD);
     if (T >= A.ntiles) return;
@@ -510,8 +513,13 @@ __global__ __launch_bounds__(TXH * NW) void yee_fused(YeeArgs<R> A) {
         tx = t.x; ty = t.y; zc = t.z; b = t.w;
     } else {
         int r = T;
-        ty = A.ty0 + r % A.nyi; r /= A.nyi;
-        tx = A.tx0 + r % A.nxi; r /= A.nxi;
+        if (A.xfast) {
+            tx = A.tx0 + r % A.nxi; r /= A.nxi;
+            ty = A.ty0 + r % A.nyi; r /= A.nyi;
+        } else {
+            ty = A.ty0 + r % A.nyi; r /= A.nyi;
+            tx = A.tx0 + r % A.nxi; r /= A.nxi;
+        }
         zc = A.zcl[r % A.nzc];
         b = r / A.nzc;
     }

@@ -81,6 +81,7 @@ def _declare(lib: C.CDLL) -> None:
                                         C.c_int, C.POINTER(E)]),
         "oafd_batch_step": (C.c_int, [E, C.c_int64]),
         "oafd_batch_step_timed": (C.c_int, [E, C.c_int64, P]),
+        "oafd_batch_record": (C.c_int, [E, C.c_int64]),
         "oafd_batch_count": (C.c_int, [E]),
         "oafd_batch_step_count": (C.c_int64, [E]),
         "oafd_batch_member": (E, [E, C.c_int]),
@@ -99,7 +100,8 @@ EXPORTS = (
     "oafd_set_field", "oafd_step_count", "oafd_peak_bytes", "oafd_estimate_bytes",
     "oafd_step_timed", "oafd_describe", "oafd_plan", "oafd_copy_bandwidth",
     "oafd_debug_inject_fault", "oafd_last_error", "oafd_destroy",
-    "oafd_batch_create", "oafd_batch_step", "oafd_batch_step_timed", "oafd_batch_count",
+    "oafd_batch_create", "oafd_batch_step", "oafd_batch_step_timed", "oafd_batch_record",
+    "oafd_batch_count",
     "oafd_batch_step_count", "oafd_batch_member", "oafd_batch_describe", "oafd_batch_destroy",
 )
 

@@ -49,10 +49,12 @@ class BatchMember(HipEngine):
 
 
 class HipBatch:
-    """Problems ``cs`` stepped in lockstep in one persistent launch per step."""
+    """Problems ``cs`` stepped in lockstep in one persistent launch per step; with
+    ``record_steps`` every member records its probe series over the first ``record_steps``
+    steps (whole 64-step chunks)."""
 
     def __init__(self, cs: list[Compiled], dtype: str = "float32", max_steps: int = 200_000,
-                 use_graph: bool = True) -> None:
+                 use_graph: bool = True, record_steps: int = 0) -> None:
         if dtype not in _lib.DTYPES:
             raise ValueError(f"dtype {dtype!r} is not one of {sorted(_lib.DTYPES)}")
         if not cs:
@@ -73,7 +75,11 @@ class HipBatch:
                                              int(use_graph), C.byref(h)), "oafd_batch_create")
         del keeps
         self._h = h
+        if record_steps > 0:
+            _lib.check(self._L.oafd_batch_record(h, int(record_steps)), "oafd_batch_record")
         self.members = [BatchMember(self, i, c, kinds[i]) for i, c in enumerate(cs)]
+        for m in self.members:
+            m.record_series = record_steps > 0
 
     def __del__(self) -> None:
         h = getattr(self, "_h", None)

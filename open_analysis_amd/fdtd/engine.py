@@ -293,7 +293,7 @@ class HipEngine:
         out: dict[str, np.ndarray] = {}
         if not self.record_series:
             return out
-        steps = self.step_count
+        steps = int(self._L.oafd_series_len(self._h))
         for kk, kind in enumerate(("E", "H")):
             plist = self._probes[kind]
             if not plist or steps == 0:

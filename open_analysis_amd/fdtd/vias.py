@@ -71,11 +71,14 @@ def random_spec(rng: np.random.Generator, small: bool = False) -> ViaSpec:
         h_top=h_top, core=int(rng.integers(6, max(core_max, 7))), h_bot=h_bot, npml=npml)
 
 
-def build(spec: ViaSpec, freqs: np.ndarray | None = None) -> Model:
+def build(spec: ViaSpec, freqs: np.ndarray | None = None, pulse: str = "gauss_mod") -> Model:
+    """``pulse``: ``gauss_mod`` (10.5 GHz centre, 21 GHz bandwidth) or ``gauss`` (baseband,
+    -20 dB at 21 GHz, half as long)."""
     s = spec
     g = Grid(np.arange(s.nx + 1) * D, np.arange(s.ny + 1) * D, np.arange(s.nz + 1) * D)
     f = np.linspace(1e9, 20e9, 96) if freqs is None else np.asarray(freqs)
-    m = Model(g, Waveform("gauss_mod", 10.5e9, 21e9), f, walls=("cpml",) * 6, courant=0.99)
+    wf = Waveform("gauss_mod", 10.5e9, 21e9) if pulse == "gauss_mod" else Waveform("gauss", bandwidth=21e9)
+    m = Model(g, wf, f, walls=("cpml",) * 6, courant=0.99)
     npml = s.npml
     m.cpml = CpmlSpec(cells=npml, m=3.0, r0=1e-6)
     # z levels (cells): bottom traces, lower plane, upper plane, top traces
@@ -122,9 +125,9 @@ def build(spec: ViaSpec, freqs: np.ndarray | None = None) -> Model:
     return m
 
 
-def geometry_set(n: int, seed: int = 11, freqs: np.ndarray | None = None, small: bool = False
-                 ) -> tuple[list[Model], list[ViaSpec]]:
+def geometry_set(n: int, seed: int = 11, freqs: np.ndarray | None = None, small: bool = False,
+                 pulse: str = "gauss_mod") -> tuple[list[Model], list[ViaSpec]]:
     """``n`` seeded transitions (uneven grids)."""
     rng = np.random.default_rng(seed)
     specs = [random_spec(rng, small) for _ in range(n)]
-    return [build(s, freqs) for s in specs], specs
+    return [build(s, freqs, pulse) for s in specs], specs

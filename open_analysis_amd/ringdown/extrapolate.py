@@ -1,0 +1,301 @@
+"""Early stopping with the learned ringdown extrapolator.
+
+A run is stepped to ``K`` steps, the network reads the port voltages so far and predicts each
+port's waveform from ``K`` on as 16 decaying complex pole pairs; the run continues for a check
+window of ``W`` steps, and the prediction must reproduce the window (residual-energy check).
+When it does, the S-parameters are the running DFT at ``K + W`` plus the closed-form DFT of the
+predicted waveform from ``K + W`` on; when it does not, the run falls back to its full length.
+
+Sample layout (per (geometry, driven port)): 32 input channels of ``T = K / DEC`` samples
+(channels 0-3 the four port voltages, decimated and divided by the sample's largest |V|;
+4-7 a one-hot of the driven port; 8-19 the normalised geometry descriptor), the amplitude
+scale, and per port and frequency the factor ``kappa = cos(w dt/2) / (sqrt(R_i) a_j(f))`` that
+turns a port-voltage DFT into S-parameter units (run.py:218-246 of the reference for the
+normalisation).
+"""
+
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+from typing import Any
+
+import numpy as np
+
+from ..fdtd.batch import HipBatch
+from ..fdtd.model import Compiled, Model, compile_model
+from ..fdtd.run import port_voltage_factor, source_spectrum
+from .net import NPOLE, NPORT, BatchArrays, RingNet
+
+DEC = 16          # decimation of the input waveforms
+SIG0_STEPS = 400  # decay-rate scale: softplus(u) / (SIG0_STEPS dt)
+FMAX_FACTOR = 1.5  # pole frequencies up to 1.5 x the band's top
+
+
+@dataclass
+class RunSet:
+    """Waveforms and DFT snapshots of a set of batched runs (one row per driven port)."""
+
+    K: int
+    W: int
+    n_full: int
+    dt: float
+    freqs: np.ndarray                 # (F,)
+    v: np.ndarray                     # (n, 4, K + W) port voltages
+    dft_k: np.ndarray                 # (n, 4, F) S-parameter units, DFT to K steps
+    dft_kw: np.ndarray                # (n, 4, F) to K + W
+    dft_full: np.ndarray              # (n, 4, F) to n_full
+    kappa: np.ndarray                 # (n, 4, F)
+    driven: np.ndarray                # (n,) driven port index
+    geom: np.ndarray                  # (n, 12) descriptor
+    geometry: np.ndarray              # (n,) geometry index
+    wall_s: dict[str, float] = field(default_factory=dict)
+
+
+def _port_dfts(member: Any, c: Compiled) -> dict[str, np.ndarray]:
+    probes = member.results()
+    out = {}
+    for q in c.ports:
+        parts = q.get("probe_parts") or [f"V:{q['name']}"]
+        out[q["name"]] = sum(probes[p] for p in parts)
+    return out
+
+
+def _port_series(member: Any, c: Compiled) -> dict[str, np.ndarray]:
+    ser = member.series()
+    out = {}
+    for q in c.ports:
+        parts = q.get("probe_parts") or [f"V:{q['name']}"]
+        out[q["name"]] = sum(ser[p] for p in parts)
+    return out
+
+
+def simulate(models: list[Model], descriptors: np.ndarray, K: int, W: int, n_full: int,
+             dtype: str = "float32", chunk: int = 64, log: Any = None) -> RunSet:
+    """Run every model (4 ports, every port driven once) in batches of ``chunk`` geometries,
+    recording the port voltages over the first ``K + W`` steps and the running DFT at ``K``,
+    ``K + W`` and ``n_full``."""
+    rows: dict[str, list] = {k: [] for k in ("v", "dk", "dkw", "df", "kap", "drv", "geo", "gi")}
+    walls = {"compile": 0.0, "fdtd": 0.0}
+    dt = freqs = None
+    for g0 in range(0, len(models), chunk):
+        t0 = time.perf_counter()
+        ms = models[g0:g0 + chunk]
+        cs = [compile_model(m) for m in ms]
+        walls["compile"] += time.perf_counter() - t0
+        t0 = time.perf_counter()
+        batch = HipBatch(cs, dtype, max_steps=n_full, record_steps=K + W)
+        batch.step(K)
+        snap_k = [_port_dfts(m, c) for m, c in zip(batch.members, cs, strict=True)]
+        batch.step(W)
+        snap_kw = [_port_dfts(m, c) for m, c in zip(batch.members, cs, strict=True)]
+        batch.step(n_full - K - W)
+        snap_n = [_port_dfts(m, c) for m, c in zip(batch.members, cs, strict=True)]
+        series = [_port_series(m, c) for m, c in zip(batch.members, cs, strict=True)]
+        del batch
+        walls["fdtd"] += time.perf_counter() - t0
+        for gi, (m, c) in enumerate(zip(ms, cs, strict=True)):
+            names = [p.name for p in m.ports]
+            if len(names) != NPORT:
+                raise ValueError("the extrapolator expects four ports")
+            dt, freqs = c.dt, np.asarray(m.frequencies, dtype=np.float64)
+            r = np.array([float(q["r_ohm"]) for q in c.ports])
+            src = source_spectrum(c, n_full)
+            fac = port_voltage_factor(c.dt, freqs)
+            for b, j in enumerate(m.excited_indices()):
+                a_j = src[j] / (2.0 * np.sqrt(r[j]))
+                kap = np.stack([fac / (np.sqrt(r[i]) * a_j) for i in range(NPORT)])
+                rows["kap"].append(kap)
+                rows["dk"].append(np.stack([snap_k[gi][n][b] for n in names]) * kap)
+                rows["dkw"].append(np.stack([snap_kw[gi][n][b] for n in names]) * kap)
+                rows["df"].append(np.stack([snap_n[gi][n][b] for n in names]) * kap)
+                rows["v"].append(np.stack([series[gi][n][b, :K + W] for n in names]))
+                rows["drv"].append(j)
+                rows["geo"].append(descriptors[g0 + gi])
+                rows["gi"].append(g0 + gi)
+        if log is not None:
+            print(f"[ringdown] simulated {min(g0 + chunk, len(models))}/{len(models)} geometries",
+                  file=log, flush=True)
+    return RunSet(K=K, W=W, n_full=n_full, dt=float(dt), freqs=freqs,
+                  v=np.asarray(rows["v"]), dft_k=np.asarray(rows["dk"]),
+                  dft_kw=np.asarray(rows["dkw"]), dft_full=np.asarray(rows["df"]),
+                  kappa=np.asarray(rows["kap"]), driven=np.asarray(rows["drv"]),
+                  geom=np.asarray(rows["geo"], dtype=np.float64),
+                  geometry=np.asarray(rows["gi"]), wall_s=walls)
+
+
+@dataclass
+class Normaliser:
+    """Descriptor mean and spread (fixed from the training set)."""
+
+    mean: np.ndarray
+    std: np.ndarray
+
+    @staticmethod
+    def fit(geom: np.ndarray) -> Normaliser:
+        return Normaliser(geom.mean(axis=0), np.maximum(geom.std(axis=0), 1e-9))
+
+
+def time_len(K: int) -> int:
+    return K // DEC
+
+
+def scales(rs: RunSet) -> tuple[float, float]:
+    """(sig0, wmax) of the pole parameterisation."""
+    return 1.0 / (SIG0_STEPS * rs.dt), 2.0 * np.pi * FMAX_FACTOR * float(rs.freqs.max())
+
+
+def inputs(rs: RunSet, idx: np.ndarray, norm: Normaliser) -> tuple[np.ndarray, np.ndarray]:
+    """Input channels (32, len(idx) * T) and amplitude scales for rows ``idx``."""
+    T = time_len(rs.K)
+    U = np.zeros((32, len(idx) * T), np.float32)
+    amp = np.zeros(len(idx), np.float32)
+    for q, i in enumerate(idx):
+        v = rs.v[i, :, :rs.K]
+        a = float(np.abs(v).max()) or 1.0
+        amp[q] = a
+        # sample n at t = (n + 1) dt; keep the last sample before K in every decimated block
+        U[0:4, q * T:(q + 1) * T] = v[:, DEC - 1::DEC][:, :T] / a
+        U[4 + rs.driven[i], q * T:(q + 1) * T] = 1.0
+        U[8:20, q * T:(q + 1) * T] = ((rs.geom[i] - norm.mean) / norm.std)[:, None]
+    return U, amp
+
+
+def batch_arrays(rs: RunSet, idx: np.ndarray, norm: Normaliser) -> BatchArrays:
+    """Training batch: the target is the tail from ``K`` in S-parameter units."""
+    U, amp = inputs(rs, idx, norm)
+    sig0, wmax = scales(rs)
+    return BatchArrays(U=U, amp=amp, kap=rs.kappa[idx], tgt=rs.dft_full[idx] - rs.dft_k[idx],
+                       t0=np.full(len(idx), (rs.K + 1) * rs.dt), omega=2 * np.pi * rs.freqs,
+                       dt=rs.dt, sig0=sig0, wmax=wmax)
+
+
+def poles_from_raw(raw: np.ndarray, amp: np.ndarray, sig0: float, wmax: float):
+    """(s, c) per (sample, port, pole) from the network outputs (256, S), in FP64."""
+    S = raw.shape[1]
+    r = raw.astype(np.float64).T.reshape(S, NPORT, NPOLE, 4)
+    u, v = r[..., 0], r[..., 1]
+    sp = np.where(u > 30.0, u, np.log1p(np.exp(np.minimum(u, 30.0))))
+    s = -sp * sig0 + 1j * wmax / (1.0 + np.exp(-v))
+    c = (r[..., 2] + 1j * r[..., 3]) * amp.astype(np.float64)[:, None, None]
+    return s, c
+
+
+def waveform(s: np.ndarray, c: np.ndarray, tau: np.ndarray) -> np.ndarray:
+    """sum_m 2 Re(c e^{s tau}) per (sample, port) at times ``tau`` (len L) -> (S, 4, L)."""
+    e = np.exp(s[..., None] * tau[None, None, None, :])
+    return 2.0 * np.real(np.sum(c[..., None] * e, axis=2))
+
+
+def tail_dft(s: np.ndarray, c: np.ndarray, t_first: float, dt: float, omega: np.ndarray,
+             shift: float = 0.0) -> np.ndarray:
+    """DFT of the pole waveform from ``t_first + shift`` on (samples every dt), (S, 4, F):
+    sum_m c' g(s) + conj(c') g(conj s), c' = c e^{s shift}."""
+    cp = c * np.exp(s * shift)
+    w = omega[None, None, None, :]
+    t0 = t_first + shift
+    ph = np.exp(-1j * w * t0) * dt
+    ga = ph / (1.0 - np.exp((s[..., None] - 1j * w) * dt))
+    gb = ph / (1.0 - np.exp((np.conj(s)[..., None] - 1j * w) * dt))
+    return np.sum(cp[..., None] * ga + np.conj(cp)[..., None] * gb, axis=2)
+
+
+@dataclass
+class Prediction:
+    s_params: np.ndarray   # (n, 4, F) driven column, S-parameter units
+    fallback: np.ndarray   # (n,) bool: residual check failed, full-run value used
+    residual: np.ndarray   # (n,) residual energy of the check window / signal energy
+    steps: np.ndarray      # (n,) steps the answer needed
+
+
+def predict(net: RingNet, rs: RunSet, idx: np.ndarray, norm: Normaliser, threshold: float = 0.05
+            ) -> Prediction:
+    """Learned extrapolation for rows ``idx`` (at most the network's batch size at a time),
+    with the residual-energy check over the window [K, K + W) and the full run as fallback."""
+    S = net.S
+    sig0, wmax = scales(rs)
+    out = np.zeros((len(idx), NPORT, rs.freqs.size), np.complex128)
+    fb = np.zeros(len(idx), bool)
+    res = np.zeros(len(idx))
+    steps = np.zeros(len(idx), np.int64)
+    tau = np.arange(rs.W) * rs.dt
+    for q0 in range(0, len(idx), S):
+        sel = idx[q0:q0 + S]
+        pad = np.concatenate([sel, np.repeat(sel[-1:], S - len(sel))])
+        net.set_batch(batch_arrays(rs, pad, norm))
+        _, raw = net.forward(raw=True)
+        amp = inputs(rs, pad, norm)[1]
+        s, c = poles_from_raw(raw, amp, sig0, wmax)
+        y = waveform(s, c, tau)[:len(sel)]
+        v_win = rs.v[sel, :, rs.K:rs.K + rs.W]
+        e_sig = np.sum(rs.v[sel, :, :rs.K] ** 2, axis=(1, 2))
+        r = np.sum((v_win - y) ** 2, axis=(1, 2)) / np.maximum(e_sig, 1e-300)
+        tail = tail_dft(s, c, (rs.K + 1) * rs.dt, rs.dt, 2 * np.pi * rs.freqs,
+                        shift=rs.W * rs.dt)[:len(sel)]
+        learned = rs.dft_kw[sel] + rs.kappa[sel] * tail
+        bad = ~(r <= threshold)
+        out[q0:q0 + len(sel)] = np.where(bad[:, None, None], rs.dft_full[sel], learned)
+        fb[q0:q0 + len(sel)] = bad
+        res[q0:q0 + len(sel)] = r
+        steps[q0:q0 + len(sel)] = np.where(bad, rs.n_full, rs.K + rs.W)
+    return Prediction(out, fb, res, steps)
+
+
+def pencil(rs: RunSet, idx: np.ndarray) -> np.ndarray:
+    """The classical alternative at the same step count: the matrix-pencil tail
+    (fdtd/tail.py) fitted on the K + W recorded samples of each port."""
+    from ..fdtd.tail import extrapolate_dft, fit_tail
+
+    f_band = float(rs.freqs.max())
+    out = np.array(rs.dft_kw[idx], copy=True)
+    n = rs.K + rs.W
+    for q, i in enumerate(idx):
+        for p in range(NPORT):
+            fit = fit_tail(rs.v[i, p, :n], rs.dt, rs.dt, f_band=f_band, window=n)
+            if fit is not None:
+                out[q, p] += rs.kappa[i, p] * extrapolate_dft(fit, rs.dt, (n + 1) * rs.dt, rs.freqs)
+    return out
+
+
+def db_error(s: np.ndarray, ref: np.ndarray, floor_db: float = -40.0) -> np.ndarray:
+    """Worst |dB| difference per row over entries above ``floor_db`` in the reference: (n, 4)."""
+    d = np.abs(20 * np.log10(np.abs(s)) - 20 * np.log10(np.abs(ref)))
+    keep = 20 * np.log10(np.abs(ref)) > floor_db
+    return np.where(keep, d, 0.0).max(axis=2)
+
+
+def train(net: RingNet, rs: RunSet, idx: np.ndarray, norm: Normaliser, epochs: int,
+          lr: float = 1e-3, seed: int = 0, log: Any = None, val: np.ndarray | None = None
+          ) -> list[dict[str, float]]:
+    """Adam with a cosine schedule over shuffled batches of the rows ``idx``."""
+    rng = np.random.default_rng(seed)
+    S = net.S
+    per_epoch = max(1, len(idx) // S)
+    total = epochs * per_epoch
+    hist = []
+    step = 0
+    net.reset_optimizer()
+    for ep in range(epochs):
+        order = rng.permutation(idx)
+        losses = []
+        for q in range(per_epoch):
+            sel = order[q * S:(q + 1) * S]
+            if len(sel) < S:
+                sel = np.concatenate([sel, rng.choice(idx, S - len(sel))])
+            net.set_batch(batch_arrays(rs, sel, norm))
+            losses.append(net.forward())
+            net.backward()
+            step += 1
+            rate = lr * 0.5 * (1.0 + np.cos(np.pi * step / total))
+            net.adam(rate, step)
+        rec = {"epoch": ep, "loss": float(np.mean(losses))}
+        if val is not None and len(val) >= S:
+            vl = []
+            for q in range(len(val) // S):
+                net.set_batch(batch_arrays(rs, val[q * S:(q + 1) * S], norm))
+                vl.append(net.forward())
+            rec["val_loss"] = float(np.mean(vl))
+        hist.append(rec)
+        if log is not None:
+            print(f"[ringdown] epoch {ep} {rec}", file=log, flush=True)
+    return hist

@@ -1,0 +1,154 @@
+"""The ringdown network on the GPU (``liboaring_hip.so``): parameters, one batch, forward,
+backward and Adam, all in device memory; the host only moves batches and parameters."""
+
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+
+CH, CIN, NL, NF, NH, NO, NPOLE, NPORT, LAST = 64, 32, 6, 512, 256, 256, 16, 4, 8
+OFF_WIN = 0
+OFF_BIN = OFF_WIN + CH * CIN
+OFF_WL = OFF_BIN + CH
+LSTRIDE = CH * 2 * CH + CH
+OFF_W1 = OFF_WL + NL * LSTRIDE
+OFF_B1 = OFF_W1 + NH * NF
+OFF_W2 = OFF_B1 + NH
+OFF_B2 = OFF_W2 + NO * NH
+NPARAM = OFF_B2 + NO
+RECEPTIVE = 2 ** NL  # time samples seen by one output position
+
+
+def init_params(seed: int = 0) -> np.ndarray:
+    """Scaled uniform weights (variance 1/fan_in, tanh gain), zero biases; the output layer
+    starts small with its pole frequencies spread over the band (v biases at evenly spaced
+    logits) so every pole has a gradient from the start."""
+    rng = np.random.default_rng(seed)
+    p = np.zeros(NPARAM, np.float32)
+
+    def uni(n_out: int, n_in: int, gain: float = 1.0) -> np.ndarray:
+        a = gain * np.sqrt(3.0 / n_in)
+        return rng.uniform(-a, a, size=(n_out, n_in)).astype(np.float32)
+
+    p[OFF_WIN:OFF_BIN] = uni(CH, CIN).ravel()
+    for l in range(NL):
+        o = OFF_WL + l * LSTRIDE
+        p[o:o + 2 * CH * CH] = uni(CH, 2 * CH, 0.5).ravel()
+    p[OFF_W1:OFF_B1] = uni(NH, NF).ravel()
+    p[OFF_W2:OFF_B2] = uni(NO, NH, 0.05).ravel()
+    b2 = np.zeros((NPORT, NPOLE, 4), np.float32)
+    frac = (np.arange(NPOLE) + 0.5) / NPOLE
+    b2[:, :, 1] = np.log(frac / (1.0 - frac))
+    b2[:, :, 0] = 0.0
+    p[OFF_B2:NPARAM] = b2.ravel()
+    return p
+
+
+@dataclass
+class BatchArrays:
+    """One training or inference batch in the layout of include/oaring.h."""
+
+    U: np.ndarray      # (32, S*T) float32
+    amp: np.ndarray    # (S,) float32
+    kap: np.ndarray    # (S, 4, F) complex
+    tgt: np.ndarray    # (S, 4, F) complex
+    t0: np.ndarray     # (S,)
+    omega: np.ndarray  # (F,) float64
+    dt: float
+    sig0: float
+    wmax: float
+
+
+def _cplx(a: np.ndarray) -> np.ndarray:
+    return np.ascontiguousarray(np.stack([a.real, a.imag], axis=-1), dtype=np.float32)
+
+
+class RingNet:
+    """Device-resident network for batches of ``samples`` sequences of ``time_len`` samples
+    and ``n_freq`` target frequencies."""
+
+    def __init__(self, samples: int, time_len: int, n_freq: int, params: np.ndarray | None = None,
+                 seed: int = 0) -> None:
+        self.S, self.T, self.F = int(samples), int(time_len), int(n_freq)
+        self._L = _lib.lib()
+        if int(self._L.oaring_param_count()) != NPARAM:
+            raise RuntimeError("liboaring_hip.so has a different parameter layout")
+        h = C.c_void_p()
+        _lib.check(self._L.oaring_create(self.S, self.T, self.F, C.byref(h)), "oaring_create")
+        self._h = h
+        self.set_params(init_params(seed) if params is None else params)
+
+    def __del__(self) -> None:
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._L.oaring_destroy(h)
+            self._h = None
+
+    def set_params(self, p: np.ndarray) -> None:
+        a = np.ascontiguousarray(p, dtype=np.float32)
+        _lib.check(self._L.oaring_set_params(self._h, a.ctypes.data, a.size), "oaring_set_params")
+
+    def params(self) -> np.ndarray:
+        out = np.zeros(NPARAM, np.float32)
+        _lib.check(self._L.oaring_get_params(self._h, out.ctypes.data, out.size), "oaring_get_params")
+        return out
+
+    def grads(self) -> np.ndarray:
+        out = np.zeros(NPARAM, np.float32)
+        _lib.check(self._L.oaring_get_grads(self._h, out.ctypes.data, out.size), "oaring_get_grads")
+        return out
+
+    def reset_optimizer(self) -> None:
+        _lib.check(self._L.oaring_reset_adam(self._h), "oaring_reset_adam")
+
+    def set_batch(self, b: BatchArrays) -> None:
+        if b.U.shape != (CIN, self.S * self.T) or b.kap.shape != (self.S, NPORT, self.F):
+            raise ValueError("batch shape does not match the network")
+        self._keep = [np.ascontiguousarray(b.U, dtype=np.float32),
+                      np.ascontiguousarray(b.amp, dtype=np.float32), _cplx(b.kap), _cplx(b.tgt),
+                      np.ascontiguousarray(b.t0, dtype=np.float32),
+                      np.ascontiguousarray(b.omega, dtype=np.float64)]
+        k = self._keep
+        _lib.check(self._L.oaring_set_batch(self._h, k[0].ctypes.data, k[1].ctypes.data,
+                                            k[2].ctypes.data, k[3].ctypes.data, k[4].ctypes.data,
+                                            k[5].ctypes.data, float(b.dt), float(b.sig0),
+                                            float(b.wmax)), "oaring_set_batch")
+
+    def forward(self, pred: bool = False, raw: bool = False):
+        """Mean loss, and optionally the predicted tails (S, 4, F) complex in S-parameter units
+        and the raw outputs (256, S)."""
+        loss = C.c_float()
+        pr = np.zeros((self.S, NPORT, self.F, 2), np.float32) if pred else None
+        rw = np.zeros((NO, self.S), np.float32) if raw else None
+        _lib.check(self._L.oaring_forward(self._h, C.byref(loss),
+                                          pr.ctypes.data if pred else None,
+                                          rw.ctypes.data if raw else None), "oaring_forward")
+        out = [float(loss.value)]
+        if pred:
+            out.append(pr[..., 0] + 1j * pr[..., 1])
+        if raw:
+            out.append(rw)
+        return out[0] if len(out) == 1 else tuple(out)
+
+    def backward(self) -> None:
+        _lib.check(self._L.oaring_backward(self._h), "oaring_backward")
+
+    def adam(self, lr: float, step: int, b1: float = 0.9, b2: float = 0.999,
+             eps: float = 1e-8) -> None:
+        _lib.check(self._L.oaring_adam(self._h, lr, b1, b2, eps, int(step)), "oaring_adam")
+
+
+def gemm_check(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    """``a @ b`` through the network's BF16 MFMA product kernel."""
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    b = np.ascontiguousarray(b, dtype=np.float32)
+    m, k = a.shape
+    n = b.shape[1]
+    out = np.zeros((m, n), np.float32)
+    _lib.check(_lib.lib().oaring_gemm_check(m, n, k, a.ctypes.data, b.ctypes.data,
+                                            out.ctypes.data), "oaring_gemm_check")
+    return out

@@ -177,3 +177,22 @@ def test_every_tile_shape_matches_golden(name: str, variant: int, monkeypatch) -
     for k, v in eng.results().items():
         ref = gold["dft_" + k.replace(":", "_")]
         assert np.abs(v - ref).max() <= 1e-4 * np.abs(ref).max(), k
+
+
+@pytest.mark.parametrize("variant", [0, 3, 6])
+def test_tile_launch_order_is_bitwise_neutral(variant: int, monkeypatch) -> None:
+    """The x-fastest and y-fastest tile launch orders (an autotuner choice) give the same bits."""
+    _say(f"[order] variant={variant}")
+    model, steps = build("interior_block")
+    c = compile_model(model)
+    out = []
+    for xf in (0, 1):
+        monkeypatch.setenv("OAFD_YEE_VARIANT", f"{variant},{xf}")
+        eng = HipEngine(c, "float32", max_steps=steps)
+        assert eng.describe()["variant"]["x_fastest"] == xf
+        eng.step(steps)
+        out.append((eng.get_fields(), eng.results()))
+    for k in out[0][0]:
+        assert np.array_equal(out[0][0][k], out[1][0][k]), k
+    for k in out[0][1]:
+        assert np.array_equal(out[0][1][k], out[1][1][k]), k
