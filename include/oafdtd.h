@@ -17,6 +17,7 @@
  *   oafd_get_field       CupyEngine.get_fields         engine.py:478-484
  *   oafd_peak_bytes      CupyEngine.peak_bytes         engine.py:505-506
  *   oafd_estimate_bytes  estimate_bytes                engine.py:29-40
+ *   oafd_sparams         source_spectrum / sparams     run.py:36-43, 203-246 (host in the reference)
  *   oafd_device_count/   oa_accel probe / select       accel/probe.py:92, accel/policy.py:147
  *   oafd_device_info
  *   oafd_debug_inject_fault  CupyBackend.inject_fault  accel/backend.py:217-219
@@ -126,6 +127,13 @@ int oafd_step(oafd_engine* e, int64_t n);
 int oafd_energy(oafd_engine* e, double* out_joules);
 /* out: (n_freq, batch, n_probe[kind], 2) float64 (re, im) */
 int oafd_results(oafd_engine* e, int kind, double* out);
+/* S-parameters of the steps done so far, in FP64 on the device (replaces the host
+ * source_spectrum / port_voltage_factor / sparams, run.py:36-43, 203-246): port i is fed by
+ * injection channel chan[i], its voltage is the sum of E probes pidx[pstart[i] .. pstart[i+1]),
+ * its resistance r[i]; batch member k drives port driven[k]. The source DFT is summed with Kahan
+ * compensation from the device wave table. out: (n_ports, batch, n_freq, 2) = S[i, driven[k]]. */
+int oafd_sparams(oafd_engine* e, int n_ports, const int32_t* chan, const int32_t* pstart, const int32_t* pidx,
+                 const double* r, const int32_t* driven, double* out);
 /* steps recorded in the probe series */
 int64_t oafd_series_len(const oafd_engine* e);
 /* out: (steps, batch, n_probe[kind]) float64; cap = capacity in elements */
@@ -141,8 +149,8 @@ int64_t oafd_estimate_bytes(const oafd_desc* d, int dtype);
  * out[2] = fused launches, out[3] = summed ms of the face/source/probe kernels. */
 int oafd_step_timed(oafd_engine* e, int64_t n, double* out);
 /* JSON text: the fused-step tile shape the autotuner chose and the timing of every candidate.
- * OAFD_YEE_VARIANT=<n> or <n>,<x> in the environment forces tile shape n (tests run each one)
- * and, with x = 1, the x-fastest tile launch order. */
+ * OAFD_YEE_VARIANT=<n>[,<x>[,<c>]] in the environment forces tile shape n (tests run each
+ * one), with x = 1 the x-fastest tile launch order, and c interior z chunks. */
 int oafd_describe(const oafd_engine* e, char* buf, int len);
 /* Host only (no device): the fused step's tile plan for a grid. npml = slab widths
  * (xlo xhi ylo yhi zlo zhi); zslab = 1 when no z wall is PMC; kb = z chunk boundaries

@@ -214,6 +214,65 @@ __global__ __launch_bounds__(256) void dft_accum(const double* ring, int BP, int
     comp[2 * t + 1] = ci;
 }
 
+// ---- S-parameters in FP64 on the device (run.py:36-43 source_spectrum, :203-246 sparams)
+// source DFT a_c(f) = sum_{n < steps} s_c((n + 1/2) dt) e^{-j w (n + 1/2) dt} dt, Kahan-compensated;
+// one thread per (channel, frequency)
+__global__ __launch_bounds__(256) void source_dft(const double* wave, int nch, long long max_steps,
+                                                  long long steps, const double* omega, int nf, double dt,
+                                                  double* out) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= nch * nf) return;
+    const int ch = t / nf, f = t - (t / nf) * nf;
+    const double w = omega[f];
+    double sr = 0.0, si = 0.0, cr = 0.0, ci = 0.0;
+    for (long long n = 0; n < steps; ++n) {
+        const double v = wave[(long long)ch * max_steps + n] * dt;
+        double sn, cs;
+        sincos(-w * ((double)n + 0.5) * dt, &sn, &cs);
+        const double yr = v * cs - cr;
+        const double tr = sr + yr;
+        cr = (tr - sr) - yr;
+        sr = tr;
+        const double yi = v * sn - ci;
+        const double ti = si + yi;
+        ci = (ti - si) - yi;
+        si = ti;
+    }
+    out[2 * t] = sr;
+    out[2 * t + 1] = si;
+}
+
+// S[i][k][f] for port i, batch member k (driving port drv[k]) from the E-probe DFT accumulators
+// acc (nf, B, nP) complex: V_i = sum of the port's probe parts; a_j = src[chan_j] / (2 sqrt(R_j));
+// S = (V_i cos(w dt / 2) / sqrt(R_i) - delta_ij a_j) / a_j
+__global__ __launch_bounds__(256) void sparams_kernel(const double* acc, int B, int nP, int nf, const double* omega,
+                                                      double dt, const double* src, int n_ports, const int* chan,
+                                                      const int* pstart, const int* pidx, const double* r,
+                                                      const int* drv, double* out) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= n_ports * B * nf) return;
+    const int f = t % nf, k = (t / nf) % B, i = t / (nf * B);
+    const int j = drv[k];
+    double vr = 0.0, vi = 0.0;
+    for (int q = pstart[i]; q < pstart[i + 1]; ++q) {
+        const long long a = (((long long)f * B + k) * nP + pidx[q]) * 2;
+        vr += acc[a];
+        vi += acc[a + 1];
+    }
+    const double fac = cos(0.5 * omega[f] * dt);
+    const double sj = sqrt(r[j]);
+    const double ar = src[2 * ((long long)chan[j] * nf + f)] / (2.0 * sj);
+    const double ai = src[2 * ((long long)chan[j] * nf + f) + 1] / (2.0 * sj);
+    double br = vr * fac / sqrt(r[i]), bi = vi * fac / sqrt(r[i]);
+    if (i == j) {
+        br -= ar;
+        bi -= ai;
+    }
+    const double den = ar * ar + ai * ai;
+    out[2 * t] = (br * ar + bi * ai) / den;
+    out[2 * t + 1] = (bi * ar - br * ai) / den;
+}
+
 __global__ void advance_step(long long* step_base, int by) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *step_base += by;
 }

@@ -71,6 +71,8 @@ struct oafd_engine {
     virtual int step_timed(int64_t n, double* out) = 0;
     virtual std::string describe() const = 0;
     virtual int64_t series_len() const { return step_count; }
+    virtual int sparams(int n_ports, const int32_t* chan, const int32_t* pstart, const int32_t* pidx,
+                        const double* r, const int32_t* drv, double* out) = 0;
     int64_t step_count = 0;
     int64_t max_steps = 0;
     int64_t peak = 0;
@@ -108,6 +110,16 @@ std::vector<YeeVariant<R>> variants() {
     else
         return {variant<S, R, 4, 2>(), variant<S, R, 4, 1>(), variant<S, R, 4, 4>(), variant<S, R, 8, 1>(),
                 variant<S, R, 8, 2>(), variant<S, R, 12, 1>(), variant<S, R, 16, 1>()};
+}
+
+// OAFD_YEE_VARIANT text "<shape>[,<x fastest>[,<interior chunks>]]" into f[0..2]
+void parse_force(const char* text, int f[3]) {
+    const char* p = text;
+    for (int q = 0; q < 3 && p && *p; ++q) {
+        char* end = nullptr;
+        f[q] = (int)std::strtol(p, &end, 10);
+        p = (end && *end == ',') ? end + 1 : nullptr;
+    }
 }
 
 // Work decomposition of the fused step for one tile shape.
@@ -584,11 +596,10 @@ struct Engine final : oafd_engine {
         vars = variants<S, R>();
         const char* force = std::getenv("OAFD_YEE_VARIANT");
         const long long cells = (long long)bs * B;
-        if (force && *force) {  // "<shape>" or "<shape>,<x fastest 0|1>"
-            char* rest = nullptr;
-            const long v = std::strtol(force, &rest, 10);
-            const int xf = (rest && *rest == ',') ? (std::atoi(rest + 1) != 0) : 0;
-            plan = make_plan(std::min<int>(std::max<int>(0, (int)v), (int)vars.size() - 1), 0, xf);
+        if (force && *force) {  // "<shape>[,<x fastest 0|1>[,<interior chunks>]]"
+            int f[3] = {0, 0, 0};
+            parse_force(force, f);
+            plan = make_plan(std::min<int>(std::max<int>(0, f[0]), (int)vars.size() - 1), std::max(0, f[2]), f[1] != 0);
             return;
         }
         if (cells < (1LL << 22)) {
@@ -766,6 +777,59 @@ struct Engine final : oafd_engine {
         graph_state = 1;
     }
     int64_t series_len() const override { return series_steps; }
+
+    // S-parameters of the steps done so far, FP64 on the device (see sparams_kernel)
+    int sparams(int n_ports, const int32_t* chan, const int32_t* pstart, const int32_t* pidx, const double* r,
+                const int32_t* drv, double* out) override {
+        if (n_ports < 1 || !chan || !pstart || !pidx || !r || !drv || !out)
+            return fail(OAFD_E_INVALID, "bad S-parameter arguments");
+        if (!have_inj || !have_probe || nP[0] == 0) return fail(OAFD_E_INVALID, "no sources or voltage probes");
+        for (int i = 0; i < n_ports; ++i) {
+            if (chan[i] < 0 || chan[i] >= inj.nch || !(r[i] > 0.0)) return fail(OAFD_E_INVALID, "bad port channel or R");
+            if (pstart[i + 1] < pstart[i]) return fail(OAFD_E_INVALID, "bad port probe list");
+        }
+        for (int q = 0; q < pstart[n_ports]; ++q)
+            if (pidx[q] < 0 || pidx[q] >= nP[0]) return fail(OAFD_E_INVALID, "bad port probe index");
+        for (int k = 0; k < B; ++k)
+            if (drv[k] < 0 || drv[k] >= n_ports) return fail(OAFD_E_INVALID, "bad driven port");
+        const int nch = inj.nch;
+        std::vector<void*> tmp;
+        auto up = [&](const void* h, size_t bytes) {
+            void* d = nullptr;
+            HIP_TRY(hipMalloc(&d, std::max<size_t>(bytes, 8)));
+            tmp.push_back(d);
+            HIP_TRY(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, stream));
+            return d;
+        };
+        int rc = OAFD_OK;
+        try {
+            double* src = nullptr;
+            HIP_TRY(hipMalloc(&src, (size_t)nch * nf * 2 * sizeof(double)));
+            tmp.push_back(src);
+            hipLaunchKernelGGL(oafd::source_dft, dim3((nch * nf + 255) / 256), dim3(256), 0, stream, inj.wave, nch,
+                               (long long)inj.max_steps, (long long)step_count, omega, nf, dt, src);
+            HIP_TRY(hipGetLastError());
+            const int* dchan = (const int*)up(chan, n_ports * sizeof(int));
+            const int* dps = (const int*)up(pstart, (n_ports + 1) * sizeof(int));
+            const int* dpi = (const int*)up(pidx, std::max(pstart[n_ports], 1) * sizeof(int));
+            const double* dr = (const double*)up(r, n_ports * sizeof(double));
+            const int* ddrv = (const int*)up(drv, B * sizeof(int));
+            double* dout = nullptr;
+            HIP_TRY(hipMalloc(&dout, (size_t)n_ports * B * nf * 2 * sizeof(double)));
+            tmp.push_back(dout);
+            const int nt = n_ports * B * nf;
+            hipLaunchKernelGGL(oafd::sparams_kernel, dim3((nt + 255) / 256), dim3(256), 0, stream, acc[0], B, nP[0], nf,
+                               omega, dt, src, n_ports, dchan, dps, dpi, dr, ddrv, dout);
+            HIP_TRY(hipGetLastError());
+            HIP_TRY(hipMemcpyAsync(out, dout, (size_t)nt * 2 * sizeof(double), hipMemcpyDeviceToHost, stream));
+            HIP_TRY(hipStreamSynchronize(stream));
+        } catch (const HipFail& f) {
+            rc = fail(f.code, f.msg);
+        }
+        (void)hipStreamSynchronize(stream);
+        for (void* d : tmp) (void)hipFree(d);
+        return rc;
+    }
     void pull_series(int slot0, int count) {
         if (!record_series) return;
         HIP_TRY(hipStreamSynchronize(stream));
@@ -1145,11 +1209,10 @@ struct Batch final : BatchBase {
     // Autotuner over tile shapes and interior chunk counts, timing the persistent launch alone.
     void tune(long long cells) {
         const char* force = std::getenv("OAFD_YEE_VARIANT");
-        if (force && *force) {  // "<shape>" or "<shape>,<x fastest 0|1>"
-            char* rest = nullptr;
-            const long v = std::strtol(force, &rest, 10);
-            const int xf = (rest && *rest == ',') ? (std::atoi(rest + 1) != 0) : 0;
-            build(std::min<int>(std::max<int>(0, (int)v), (int)vars.size() - 1), 0, xf);
+        if (force && *force) {  // "<shape>[,<x fastest 0|1>[,<interior chunks>]]"
+            int f[3] = {0, 0, 0};
+            parse_force(force, f);
+            build(std::min<int>(std::max<int>(0, f[0]), (int)vars.size() - 1), std::max(0, f[2]), f[1] != 0);
         } else if (cells < (1LL << 22)) {
             build(0, 0);
         } else {
@@ -1474,6 +1537,10 @@ int oafd_describe(const oafd_engine* e, char* buf, int len) {
 }
 
 int64_t oafd_series_len(const oafd_engine* e) { return e ? e->series_len() : 0; }
+int oafd_sparams(oafd_engine* e, int n_ports, const int32_t* chan, const int32_t* pstart, const int32_t* pidx,
+                 const double* r, const int32_t* driven, double* out) {
+    ENGINE_CALL(e, return e->sparams(n_ports, chan, pstart, pidx, r, driven, out));
+}
 int64_t oafd_step_count(const oafd_engine* e) { return e ? e->step_count : 0; }
 int64_t oafd_peak_bytes(const oafd_engine* e) { return e ? e->peak : 0; }
 int64_t oafd_estimate_bytes(const oafd_desc* d, int dtype) {

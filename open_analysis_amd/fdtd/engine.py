@@ -322,6 +322,27 @@ class HipEngine:
     def peak_bytes(self) -> int:
         return int(self._L.oafd_peak_bytes(self._h))
 
+    def device_sparams(self, driven: list[int]) -> np.ndarray:
+        """``S[i, k, f]`` of port ``i`` for batch member ``k`` (which drives port
+        ``driven[k]``), normalised in FP64 on the device from the steps done so far."""
+        c = self.c
+        names = [p["name"] for p in self._probes["E"]]
+        chan, pstart, pidx, r = [], [0], [], []
+        for q in c.ports:
+            parts = q.get("probe_parts") or [f"V:{q['name']}"]
+            pidx += [names.index(p) for p in parts]
+            pstart.append(len(pidx))
+            chan.append(int(q["channel"]))
+            r.append(float(q["r_ohm"]))
+        n = len(c.ports)
+        a = [np.ascontiguousarray(x, dtype=t) for x, t in ((chan, np.int32), (pstart, np.int32),
+                                                        (pidx, np.int32), (r, np.float64),
+                                                        (driven, np.int32))]
+        out = np.zeros((n, self.B, self.c.frequencies.size, 2))
+        _lib.check(self._L.oafd_sparams(self._h, n, *[x.ctypes.data for x in a], out.ctypes.data),
+                   "oafd_sparams")
+        return out[..., 0] + 1j * out[..., 1]
+
 
 def hip_factory(c: Compiled, dtype: str, record_series: bool, max_steps: int):
     """Engine factory for :func:`run` (``engine=hip_factory``) without the device policy."""

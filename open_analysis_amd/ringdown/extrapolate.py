@@ -8,10 +8,13 @@ predicted waveform from ``K + W`` on; when it does not, the run falls back to it
 
 Sample layout (per (geometry, driven port)): 32 input channels of ``T = K / DEC`` samples
 (channels 0-3 the four port voltages, decimated and divided by the sample's largest |V|;
-4-7 a one-hot of the driven port; 8-19 the normalised geometry descriptor), the amplitude
-scale, and per port and frequency the factor ``kappa = cos(w dt/2) / (sqrt(R_i) a_j(f))`` that
-turns a port-voltage DFT into S-parameter units (run.py:218-246 of the reference for the
-normalisation).
+4-7 a one-hot of the driven port; 8-19 the normalised geometry descriptor; 20-23 the voltages
+divided by their RMS over the last ``LATE`` steps before ``K``, clipped to +-16), and the
+residue scale (that late RMS). The network is trained on the tail's port-voltage DFT over a
+grid wider than the band (so the loss constrains the whole tail waveform), each sample divided
+by its tail's RMS spectrum; S-parameters use the factor
+``kappa = cos(w dt/2) / (sqrt(R_i) a_j(f))`` from a port-voltage DFT to S units (run.py:218-246
+of the reference for the normalisation).
 """
 
 from __future__ import annotations
@@ -28,8 +31,9 @@ from ..fdtd.run import port_voltage_factor, source_spectrum
 from .net import NPOLE, NPORT, BatchArrays, RingNet
 
 DEC = 16          # decimation of the input waveforms
+LATE = 512        # steps before K whose RMS scales the residues
 SIG0_STEPS = 400  # decay-rate scale: softplus(u) / (SIG0_STEPS dt)
-FMAX_FACTOR = 1.5  # pole frequencies up to 1.5 x the band's top
+FMAX_FACTOR = 1.0  # pole frequencies up to the top of the DFT grid
 
 
 @dataclass
@@ -45,7 +49,7 @@ class RunSet:
     dft_k: np.ndarray                 # (n, 4, F) S-parameter units, DFT to K steps
     dft_kw: np.ndarray                # (n, 4, F) to K + W
     dft_full: np.ndarray              # (n, 4, F) to n_full
-    kappa: np.ndarray                 # (n, 4, F)
+    kappa: np.ndarray                 # (n, 4, F) port-voltage DFT -> S units
     driven: np.ndarray                # (n,) driven port index
     geom: np.ndarray                  # (n, 12) descriptor
     geometry: np.ndarray              # (n,) geometry index
@@ -146,26 +150,40 @@ def scales(rs: RunSet) -> tuple[float, float]:
 
 
 def inputs(rs: RunSet, idx: np.ndarray, norm: Normaliser) -> tuple[np.ndarray, np.ndarray]:
-    """Input channels (32, len(idx) * T) and amplitude scales for rows ``idx``."""
+    """Input channels (32, len(idx) * T) and residue scales for rows ``idx``."""
     T = time_len(rs.K)
     U = np.zeros((32, len(idx) * T), np.float32)
     amp = np.zeros(len(idx), np.float32)
     for q, i in enumerate(idx):
         v = rs.v[i, :, :rs.K]
-        a = float(np.abs(v).max()) or 1.0
-        amp[q] = a
+        peak = float(np.abs(v).max()) or 1.0
+        late = float(np.sqrt(np.mean(v[:, rs.K - LATE:] ** 2))) or peak * 1e-6
+        amp[q] = late
         # sample n at t = (n + 1) dt; keep the last sample before K in every decimated block
-        U[0:4, q * T:(q + 1) * T] = v[:, DEC - 1::DEC][:, :T] / a
+        vd = v[:, DEC - 1::DEC][:, :T]
+        U[0:4, q * T:(q + 1) * T] = vd / peak
         U[4 + rs.driven[i], q * T:(q + 1) * T] = 1.0
         U[8:20, q * T:(q + 1) * T] = ((rs.geom[i] - norm.mean) / norm.std)[:, None]
+        U[20:24, q * T:(q + 1) * T] = np.clip(vd / late, -16.0, 16.0)
     return U, amp
 
 
+def tail_target(rs: RunSet, idx: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    """(kap, tgt) of the training loss: the tail's port-voltage DFT from K, divided by each
+    sample's RMS over ports and frequencies."""
+    dv = (rs.dft_full[idx] - rs.dft_k[idx]) / rs.kappa[idx]
+    nu = np.sqrt(np.mean(np.abs(dv) ** 2, axis=(1, 2)))
+    nu = np.where(nu > 0, nu, 1.0)
+    kap = np.broadcast_to((1.0 / nu)[:, None, None], dv.shape).astype(np.complex128)
+    return kap, dv / nu[:, None, None]
+
+
 def batch_arrays(rs: RunSet, idx: np.ndarray, norm: Normaliser) -> BatchArrays:
-    """Training batch: the target is the tail from ``K`` in S-parameter units."""
+    """Training batch: the target is the normalised tail from ``K``."""
     U, amp = inputs(rs, idx, norm)
     sig0, wmax = scales(rs)
-    return BatchArrays(U=U, amp=amp, kap=rs.kappa[idx], tgt=rs.dft_full[idx] - rs.dft_k[idx],
+    kap, tgt = tail_target(rs, idx)
+    return BatchArrays(U=U, amp=amp, kap=kap, tgt=tgt,
                        t0=np.full(len(idx), (rs.K + 1) * rs.dt), omega=2 * np.pi * rs.freqs,
                        dt=rs.dt, sig0=sig0, wmax=wmax)
 
@@ -204,14 +222,16 @@ def tail_dft(s: np.ndarray, c: np.ndarray, t_first: float, dt: float, omega: np.
 class Prediction:
     s_params: np.ndarray   # (n, 4, F) driven column, S-parameter units
     fallback: np.ndarray   # (n,) bool: residual check failed, full-run value used
-    residual: np.ndarray   # (n,) residual energy of the check window / signal energy
+    residual: np.ndarray   # (n,) residual energy over the check window / the window's energy
     steps: np.ndarray      # (n,) steps the answer needed
 
 
-def predict(net: RingNet, rs: RunSet, idx: np.ndarray, norm: Normaliser, threshold: float = 0.05
+def predict(net: RingNet, rs: RunSet, idx: np.ndarray, norm: Normaliser, threshold: float = 0.25
             ) -> Prediction:
-    """Learned extrapolation for rows ``idx`` (at most the network's batch size at a time),
-    with the residual-energy check over the window [K, K + W) and the full run as fallback."""
+    """Learned extrapolation for rows ``idx`` (the network's batch size at a time), with the
+    residual-energy check over the window [K, K + W): the energy of (recorded - predicted)
+    waveform over the window, relative to the recorded window's energy, must not exceed
+    ``threshold``; otherwise the full-length run's value is used (fallback)."""
     S = net.S
     sig0, wmax = scales(rs)
     out = np.zeros((len(idx), NPORT, rs.freqs.size), np.complex128)
@@ -228,8 +248,8 @@ def predict(net: RingNet, rs: RunSet, idx: np.ndarray, norm: Normaliser, thresho
         s, c = poles_from_raw(raw, amp, sig0, wmax)
         y = waveform(s, c, tau)[:len(sel)]
         v_win = rs.v[sel, :, rs.K:rs.K + rs.W]
-        e_sig = np.sum(rs.v[sel, :, :rs.K] ** 2, axis=(1, 2))
-        r = np.sum((v_win - y) ** 2, axis=(1, 2)) / np.maximum(e_sig, 1e-300)
+        e_win = np.sum(v_win ** 2, axis=(1, 2))
+        r = np.sum((v_win - y) ** 2, axis=(1, 2)) / np.maximum(e_win, 1e-300)
         tail = tail_dft(s, c, (rs.K + 1) * rs.dt, rs.dt, 2 * np.pi * rs.freqs,
                         shift=rs.W * rs.dt)[:len(sel)]
         learned = rs.dft_kw[sel] + rs.kappa[sel] * tail
@@ -241,9 +261,10 @@ def predict(net: RingNet, rs: RunSet, idx: np.ndarray, norm: Normaliser, thresho
     return Prediction(out, fb, res, steps)
 
 
-def pencil(rs: RunSet, idx: np.ndarray) -> np.ndarray:
+def pencil(rs: RunSet, idx: np.ndarray, start: int = 0) -> np.ndarray:
     """The classical alternative at the same step count: the matrix-pencil tail
-    (fdtd/tail.py) fitted on the K + W recorded samples of each port."""
+    (fdtd/tail.py) fitted on the recorded samples ``start .. K + W`` of each port (``start``
+    past the excitation pulse gives it a pure ringdown to fit)."""
     from ..fdtd.tail import extrapolate_dft, fit_tail
 
     f_band = float(rs.freqs.max())
@@ -251,16 +272,21 @@ def pencil(rs: RunSet, idx: np.ndarray) -> np.ndarray:
     n = rs.K + rs.W
     for q, i in enumerate(idx):
         for p in range(NPORT):
-            fit = fit_tail(rs.v[i, p, :n], rs.dt, rs.dt, f_band=f_band, window=n)
+            fit = fit_tail(rs.v[i, p, start:n], rs.dt, (start + 1) * rs.dt, f_band=f_band,
+                           window=n - start)
             if fit is not None:
                 out[q, p] += rs.kappa[i, p] * extrapolate_dft(fit, rs.dt, (n + 1) * rs.dt, rs.freqs)
     return out
 
 
-def db_error(s: np.ndarray, ref: np.ndarray, floor_db: float = -40.0) -> np.ndarray:
-    """Worst |dB| difference per row over entries above ``floor_db`` in the reference: (n, 4)."""
+def db_error(s: np.ndarray, ref: np.ndarray, floor_db: float = -40.0,
+             band: np.ndarray | None = None) -> np.ndarray:
+    """Worst |dB| difference per row over entries above ``floor_db`` in the reference (and
+    frequencies in ``band``): (n, 4)."""
     d = np.abs(20 * np.log10(np.abs(s)) - 20 * np.log10(np.abs(ref)))
     keep = 20 * np.log10(np.abs(ref)) > floor_db
+    if band is not None:
+        keep = keep & band[None, None, :]
     return np.where(keep, d, 0.0).max(axis=2)
 
 

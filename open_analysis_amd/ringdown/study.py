@@ -1,0 +1,144 @@
+"""Train the ringdown extrapolator on batched FDTD runs of seeded via transitions and judge it
+on held-out geometries against their full-length runs.
+
+    python -m open_analysis_amd.ringdown.study --train 512 --test 128 --epochs 60 \\
+        --out ringdown.json [--params ringdown_params.npz]
+
+For every held-out (geometry, driven port) it reports the worst |dB| error over 1-20 GHz of the
+driven column of S, against the full-length run, for: the running DFT stopped at ``K + W``
+steps (no extrapolation), the matrix-pencil tail fitted on those steps (fdtd/tail.py), and the
+learned extrapolation with its residual-energy check (fallback to the full run when the check
+fails). With ``--fp16`` the held-out runs are repeated with FP16 field storage up to ``K + W``
+steps (the learned plus reduced-precision form) and judged against FP64 full-length runs.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+from typing import Any
+
+import numpy as np
+
+from ..fdtd import vias
+from . import extrapolate as X
+from .net import RingNet
+
+K_DEFAULT, W_DEFAULT, N_DEFAULT = 3008, 1024, 15000
+FREQS = np.linspace(0.25e9, 30e9, 120)  # the DFT grid (training loss); S is judged on 1-20 GHz
+BAND = (FREQS >= 1e9) & (FREQS <= 20e9)
+
+
+def geometry(n: int, seed: int):
+    models, specs = vias.geometry_set(n, seed=seed, freqs=FREQS, small=True, pulse="gauss")
+    return models, np.array([s.descriptor() for s in specs])
+
+
+def through_ports(rs: X.RunSet, idx: np.ndarray) -> np.ndarray:
+    """Index of the through port of each row: t1 <-> b1, t2 <-> b2 (ports t1, b1, t2, b2)."""
+    return rs.driven[idx] ^ 1
+
+
+def summarise(err: np.ndarray) -> dict[str, float]:
+    return {"median_db": float(np.median(err)), "p90_db": float(np.percentile(err, 90)),
+            "max_db": float(err.max()), "frac_within_0.05_db": float(np.mean(err <= 0.05))}
+
+
+def judge(rs: X.RunSet, idx: np.ndarray, s: np.ndarray, ref: np.ndarray) -> dict[str, Any]:
+    """Worst error over 1-20 GHz per row: the through entry, and every entry above -40 dB."""
+    e_all = X.db_error(s, ref, band=BAND).max(axis=1)
+    thr = through_ports(rs, idx)
+    d = np.abs(20 * np.log10(np.abs(s[np.arange(len(idx)), thr]))
+               - 20 * np.log10(np.abs(ref[np.arange(len(idx)), thr])))
+    e_thr = d[:, BAND].max(axis=1)
+    return {"through": summarise(e_thr), "all_entries": summarise(e_all)}
+
+
+def pulse_end(model) -> int:
+    """First step after the excitation pulse (two delays)."""
+    from ..fdtd.model import compile_model
+
+    wf = model.waveform
+    return int(np.ceil(2.0 * wf.delay_tau * wf.tau / compile_model(model).dt))
+
+
+def run_study(n_train: int, n_test: int, epochs: int, batch: int = 64, lr: float = 2e-3,
+              K: int = K_DEFAULT, W: int = W_DEFAULT, n_full: int = N_DEFAULT,
+              threshold: float = 0.25, fp16: bool = False, seed: int = 0, log: Any = None,
+              params: np.ndarray | None = None) -> tuple[dict[str, Any], np.ndarray]:
+    t_all = time.perf_counter()
+    m_tr, d_tr = geometry(n_train, 1000 + seed)
+    m_te, d_te = geometry(n_test, 2000 + seed)
+    rs_tr = X.simulate(m_tr, d_tr, K, W, n_full, log=log)
+    rs_te = X.simulate(m_te, d_te, K, W, n_full, log=log)
+    norm = X.Normaliser.fit(rs_tr.geom)
+    net = RingNet(batch, X.time_len(K), FREQS.size, params=params, seed=seed)
+    idx_tr = np.arange(len(rs_tr.driven))
+    idx_te = np.arange(len(rs_te.driven))
+    t0 = time.perf_counter()
+    hist = X.train(net, rs_tr, idx_tr, norm, epochs, lr=lr, seed=seed, log=log,
+                   val=idx_te) if epochs > 0 else []
+    t_train = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    pred = X.predict(net, rs_te, idx_te, norm, threshold)
+    t_pred = time.perf_counter() - t0
+    ref = rs_te.dft_full
+    report: dict[str, Any] = {
+        "K": K, "W": W, "n_full": n_full, "threshold": threshold,
+        "train_rows": int(len(idx_tr)), "test_rows": int(len(idx_te)),
+        "train_geometries": n_train, "test_geometries": n_test, "epochs": epochs,
+        "history": hist, "train_s": t_train, "predict_s": t_pred,
+        "simulate_s": {"train": rs_tr.wall_s, "test": rs_te.wall_s},
+        "truncated": judge(rs_te, idx_te, rs_te.dft_kw, ref),
+        "pencil": judge(rs_te, idx_te, X.pencil(rs_te, idx_te, start=pulse_end(m_te[0])), ref),
+        "learned": judge(rs_te, idx_te, pred.s_params, ref),
+        "fallback_rate": float(pred.fallback.mean()),
+        "residual_median": float(np.median(pred.residual)),
+        "mean_steps": float(pred.steps.mean()),
+        "step_ratio_vs_full": float(n_full / pred.steps.mean()),
+    }
+    ok = ~pred.fallback
+    report["learned_accepted_only"] = judge(rs_te, idx_te[ok], pred.s_params[ok], ref[ok]) if ok.any() else None
+    if fp16:
+        rs16 = X.simulate(m_te, d_te, K, W, K + W, dtype="float16", log=log)
+        rs64 = X.simulate(m_te, d_te, K, W, n_full, dtype="float64", log=log)
+        # FP16 early part, FP64 full-length reference; the fallback of the FP16 form is the
+        # full FP16 run, approximated here by the FP32 full run (its error is reported above)
+        rs16.dft_full = rs_te.dft_full
+        pred16 = X.predict(net, rs16, idx_te, norm, threshold)
+        report["learned_fp16_vs_fp64"] = judge(rs16, idx_te, pred16.s_params, rs64.dft_full)
+        report["fp16_fallback_rate"] = float(pred16.fallback.mean())
+        report["fp32_full_vs_fp64"] = judge(rs_te, idx_te, rs_te.dft_full, rs64.dft_full)
+        report["simulate_s"]["fp16"] = rs16.wall_s
+        report["simulate_s"]["fp64"] = rs64.wall_s
+    report["total_s"] = time.perf_counter() - t_all
+    return report, net.params()
+
+
+def main(argv: list[str] | None = None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--train", type=int, default=512)
+    ap.add_argument("--test", type=int, default=128)
+    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--threshold", type=float, default=0.25)
+    ap.add_argument("--fp16", action="store_true")
+    ap.add_argument("--out", default=None)
+    ap.add_argument("--params", default=None, help="write the trained parameters (npz)")
+    a = ap.parse_args(argv)
+    rep, p = run_study(a.train, a.test, a.epochs, lr=a.lr, threshold=a.threshold, fp16=a.fp16,
+                       log=sys.stderr)
+    text = json.dumps(rep, indent=1)
+    if a.out:
+        with open(a.out, "w") as fh:
+            fh.write(text + "\n")
+    if a.params:
+        np.savez_compressed(a.params, params=p)
+    print(text, flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -2,10 +2,10 @@
 training kernels with it; nothing in the product calls it).
 
 Forward and backward of the network in include/oaring.h, in float64, with every matrix-product
-operand rounded to BF16 (round to nearest even) exactly where the GPU kernel rounds it, so the
-GPU gradients must agree to FP32 accumulation error. With ``bf16=False`` the same code is plain
-float64 calculus; the tests pin that version with central finite differences, which pins the
-formulas the rounded version shares.
+operand rounded to BF16 (round to nearest even) exactly where the GPU kernel rounds it and every
+stored intermediate rounded to FP32, so the GPU gradients must agree to FP32 accumulation error.
+With ``rnd=False`` the same code is plain float64 calculus; the tests pin that version with
+central finite differences, which pins the formulas the rounded version shares.
 
 The network has no reference counterpart (the reference extrapolates tails only with the matrix
 pencil, packages/oa-fdtd/src/oa_fdtd/tail.py:39-98).
@@ -136,36 +136,44 @@ def pole_loss(o: np.ndarray, b: Batch):
     return loss, d.reshape(S, NO).T.copy(), y
 
 
+def f32(a: np.ndarray) -> np.ndarray:
+    return np.asarray(a, dtype=np.float32).astype(np.float64)
+
+
 def forward(p: np.ndarray, b: Batch, rnd: bool = True):
-    """Loss, predictions (S, 4, F) and the cache for :func:`backward`."""
+    """Loss, predictions (S, 4, F) and the cache for :func:`backward`. With ``rnd`` the
+    matrix-product operands are rounded to BF16 and every product, bias, tanh and residual
+    result to FP32, as the GPU kernels store them."""
     q = bf16 if rnd else (lambda a: np.asarray(a, dtype=np.float64))
+    r = f32 if rnd else (lambda a: a)
     win, bin_, layers, w1, b1, w2, b2 = _views(np.asarray(p, dtype=np.float64))
     T = b.T
-    xs = [np.tanh(q(win) @ q(b.U) + bin_[:, None])]
+    xs = [r(np.tanh(r(r(q(win) @ q(b.U)) + bin_[:, None])))]
     for l, (w, bb) in enumerate(layers):
         x = xs[-1]
         inp = np.concatenate([x, _shift(x, T, 1 << l)], axis=0)
-        xs.append(x + np.tanh(q(w) @ q(inp) + bb[:, None]))
+        xs.append(r(x + r(np.tanh(r(r(q(w) @ q(inp)) + bb[:, None])))))
     f = _feat(xs[-1], T)
-    h = np.tanh(q(w1) @ q(f) + b1[:, None])
-    o = q(w2) @ q(h) + b2[:, None]
+    h = r(np.tanh(r(r(q(w1) @ q(f)) + b1[:, None])))
+    o = r(r(q(w2) @ q(h)) + b2[:, None])
     loss, d_o, y = pole_loss(o, b)
     return loss, y, dict(xs=xs, f=f, h=h, o=o, d_o=d_o)
 
 
 def backward(p: np.ndarray, b: Batch, cache: dict, rnd: bool = True) -> np.ndarray:
     q = bf16 if rnd else (lambda a: np.asarray(a, dtype=np.float64))
+    r = f32 if rnd else (lambda a: a)
     win, bin_, layers, w1, b1, w2, b2 = _views(np.asarray(p, dtype=np.float64))
     T = b.T
-    xs, f, h, d_o = cache["xs"], cache["f"], cache["h"], cache["d_o"]
+    xs, f, h, d_o = cache["xs"], cache["f"], cache["h"], r(cache["d_o"])
     g = np.zeros(NPARAM)
     g[OFF_W2:OFF_B2] = (q(d_o) @ q(h).T).ravel()
     g[OFF_B2:NPARAM] = d_o.sum(axis=1)
-    dh = q(w2).T @ q(d_o)
-    dz1 = dh * (1.0 - h * h)
+    dh = r(q(w2).T @ q(d_o))
+    dz1 = r(dh * (1.0 - h * h))
     g[OFF_W1:OFF_B1] = (q(dz1) @ q(f).T).ravel()
     g[OFF_B1:OFF_W2] = dz1.sum(axis=1)
-    df = q(w1).T @ q(dz1)  # (512, S)
+    df = r(q(w1).T @ q(dz1))  # (512, S)
     S = f.shape[1]
     gx = np.zeros_like(xs[-1])
     gx.reshape(C, S, T)[:, :, T - LAST:] = df.reshape(LAST, C, S).transpose(1, 2, 0)
@@ -173,16 +181,16 @@ def backward(p: np.ndarray, b: Batch, cache: dict, rnd: bool = True) -> np.ndarr
         w, _ = layers[l]
         d = 1 << l
         x, xn = xs[l], xs[l + 1]
-        th = xn - x
-        dz = gx * (1.0 - th * th)
+        th = r(xn - x)
+        dz = r(gx * (1.0 - th * th))
         inp = np.concatenate([x, _shift(x, T, d)], axis=0)
         o = OFF_WL + l * LSTRIDE
         g[o:o + 2 * C * C] = (q(dz) @ q(inp).T).ravel()
         g[o + 2 * C * C:o + LSTRIDE] = dz.sum(axis=1)
         dzs = np.concatenate([dz, _unshift(dz, T, d)], axis=0)  # (128, N)
         wt = np.concatenate([w[:, :C].T, w[:, C:].T], axis=1)  # (64 ci, 128 = tap*64 + co)
-        gx = gx + q(wt) @ q(dzs)
-    dz0 = gx * (1.0 - xs[0] ** 2)
+        gx = r(gx + r(q(wt) @ q(dzs)))
+    dz0 = r(gx * (1.0 - xs[0] ** 2))
     g[OFF_WIN:OFF_BIN] = (q(dz0) @ q(b.U).T).ravel()
     g[OFF_BIN:OFF_WL] = dz0.sum(axis=1)
     return g

@@ -196,3 +196,25 @@ def test_tile_launch_order_is_bitwise_neutral(variant: int, monkeypatch) -> None
         assert np.array_equal(out[0][0][k], out[1][0][k]), k
     for k in out[0][1]:
         assert np.array_equal(out[0][1][k], out[1][1][k]), k
+
+
+@pytest.mark.parametrize("name", ["microstrip_small", "debye_sibc_line"])
+def test_device_sparams_match_host_normalisation(name: str) -> None:
+    """S normalised in FP64 on the device (Kahan source DFT from the device wave table) equals
+    the host formula on the same probe DFTs."""
+    from open_analysis_amd.fdtd import run
+    from open_analysis_amd.fdtd.engine import hip_factory
+    from open_analysis_amd.fdtd.run import host_sparams
+
+    _say(f"[sparams] case={name}")
+    model, steps = build(name)
+    if not model.ports:
+        pytest.skip("no ports")
+    res = run(model, engine=hip_factory, dtype="float32", end_db=None, max_steps=steps)
+    assert res.s_device is not None
+    host = host_sparams(model, res)
+    cols = model.excited_indices()
+    for j in cols:
+        d = np.abs(res.s_device[:, j] - host[:, j]).max() / np.abs(host[:, j]).max()
+        assert d < 1e-10, (j, d)
+    assert np.all(np.isnan(res.s_device[:, [j for j in range(len(model.ports)) if j not in cols]]))
