@@ -130,8 +130,9 @@ int oafd_results(oafd_engine* e, int kind, double* out);
 /* S-parameters of the steps done so far, in FP64 on the device (replaces the host
  * source_spectrum / port_voltage_factor / sparams, run.py:36-43, 203-246): port i is fed by
  * injection channel chan[i], its voltage is the sum of E probes pidx[pstart[i] .. pstart[i+1]),
- * its resistance r[i]; batch member k drives port driven[k]. The source DFT is summed with Kahan
- * compensation from the device wave table. out: (n_ports, batch, n_freq, 2) = S[i, driven[k]]. */
+ * its resistance r[i]; batch member k drives port driven[k] (driven holds `batch` values). The
+ * source DFT is summed with Kahan compensation from the device wave table.
+ * out: (n_ports, batch, n_freq, 2) = S[i, driven[k]]. */
 int oafd_sparams(oafd_engine* e, int n_ports, const int32_t* chan, const int32_t* pstart, const int32_t* pidx,
                  const double* r, const int32_t* driven, double* out);
 /* steps recorded in the probe series */

@@ -326,6 +326,8 @@ class HipEngine:
         """``S[i, k, f]`` of port ``i`` for batch member ``k`` (which drives port
         ``driven[k]``), normalised in FP64 on the device from the steps done so far."""
         c = self.c
+        if len(driven) != self.B:
+            raise ValueError("device_sparams needs one driven port per batch member")
         names = [p["name"] for p in self._probes["E"]]
         chan, pstart, pidx, r = [], [0], [], []
         for q in c.ports:
