@@ -23,4 +23,18 @@ __device__ __forceinline__ void stS(void* base, I i, R v) {
         static_cast<S*>(base)[i] = (S)v;
 }
 
+// Raw storage load (no conversion) and the conversion on its own: prefetched 16-bit values stay
+// packed in registers until they are used, so the load's wait is not forced at issue.
+template <typename S, typename I = long long>
+__device__ __forceinline__ S ldraw(const void* base, I i) {
+    return static_cast<const S*>(base)[i];
+}
+template <typename S, typename R>
+__device__ __forceinline__ R cvt(S v) {
+    if constexpr (__is_same(S, __bf16) || __is_same(S, _Float16))
+        return (R)(float)v;
+    else
+        return (R)v;
+}
+
 }  // namespace oafd

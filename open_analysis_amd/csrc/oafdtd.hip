@@ -216,6 +216,9 @@ struct Engine final : oafd_engine {
     bool use_graph;
     hipStream_t stream = nullptr;
     bool own_stream = true;  // false once a batch has given the engine its stream
+    // the general and z-slab launches run beside the interior launch on two forked streams
+    hipStream_t side[2] = {nullptr, nullptr};
+    hipEvent_t fork_ev = nullptr, join_ev[2] = {nullptr, nullptr};
     std::vector<void*> allocs;
     // fields [parity][comp]
     void* E[2][3] = {};
@@ -272,7 +275,14 @@ struct Engine final : oafd_engine {
         if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         if (graph) (void)hipGraphDestroy(graph);
         if (stream) (void)hipStreamSynchronize(stream);
+        for (int q = 0; q < 2; ++q)
+            if (side[q]) (void)hipStreamSynchronize(side[q]);
         for (void* p : allocs) (void)hipFree(p);
+        for (int q = 0; q < 2; ++q) {
+            if (side[q]) (void)hipStreamDestroy(side[q]);
+            if (join_ev[q]) (void)hipEventDestroy(join_ev[q]);
+        }
+        if (fork_ev) (void)hipEventDestroy(fork_ev);
         if (stream && own_stream) (void)hipStreamDestroy(stream);
     }
 
@@ -287,6 +297,13 @@ struct Engine final : oafd_engine {
         constexpr bool half = sizeof(S) == 2;
         hs = half ? ETA0 : 1.0;
         HIP_TRY(hipStreamCreate(&stream));
+        if (autotune) {  // engines of their own (batch members step through the batch kernel)
+            for (int q = 0; q < 2; ++q) {
+                HIP_TRY(hipStreamCreate(&side[q]));
+                HIP_TRY(hipEventCreateWithFlags(&join_ev[q], hipEventDisableTiming));
+            }
+            HIP_TRY(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming));
+        }
         const size_t nfield = (size_t)B * bs;
         for (int p = 0; p < 2; ++p)
             for (int c = 0; c < 3; ++c) {
@@ -572,23 +589,38 @@ struct Engine final : oafd_engine {
         oafd::YeeArgs<R> A = yee_args(parity);
         const dim3 block(oafd::TXH, plan.nw);
         const YeeVariant<R>& v = vars[plan.variant];
-        auto go = [&](const void* fn, int ntiles) {
+        auto go = [&](const void* fn, int ntiles, hipStream_t st) {
             A.ntiles = ntiles;
             void* args[] = {&A};
             const unsigned nb = (unsigned)(oafd::NXCD * ((ntiles + oafd::NXCD - 1) / oafd::NXCD));
-            HIP_TRY(hipLaunchKernel(fn, dim3(nb), block, args, 0, stream));
+            HIP_TRY(hipLaunchKernel(fn, dim3(nb), block, args, 0, st));
         };
+        // The three launches own disjoint tiles and only read the input field set, so they run
+        // concurrently: the general and z-slab launches (small grids, long tails) on two streams
+        // forked from the engine stream, joined back before the next kernel. Inside a graph
+        // capture the fork and join become graph edges.
+        const bool fork = side[0] != nullptr;
+        const bool gen = plan.n_gen > 0, zsl = plan.nzs > 0 && plan.nzc > 0;
+        if (fork && (gen || zsl)) HIP_TRY(hipEventRecord(fork_ev, stream));
+        if (gen) {
+            if (fork) HIP_TRY(hipStreamWaitEvent(side[0], fork_ev, 0));
+            go(v.general, plan.n_gen, fork ? side[0] : stream);
+            if (fork) HIP_TRY(hipEventRecord(join_ev[0], side[0]));
+        }
+        if (zsl) {
+            if (fork) HIP_TRY(hipStreamWaitEvent(side[1], fork_ev, 0));
+            A.zcl = plan.zc_zs;
+            A.nzc = plan.nzs;
+            go(v.zslab, plan.nxi * plan.nyi * plan.nzs * B, fork ? side[1] : stream);
+            if (fork) HIP_TRY(hipEventRecord(join_ev[1], side[1]));
+        }
         if (plan.nzc > 0) {
             A.zcl = plan.zc_int;
             A.nzc = plan.nzc;
-            go(v.interior, plan.nxi * plan.nyi * plan.nzc * B);
+            go(v.interior, plan.nxi * plan.nyi * plan.nzc * B, stream);
         }
-        if (plan.nzs > 0) {
-            A.zcl = plan.zc_zs;
-            A.nzc = plan.nzs;
-            go(v.zslab, plan.nxi * plan.nyi * plan.nzs * B);
-        }
-        if (plan.n_gen > 0) go(v.general, plan.n_gen);
+        if (fork && gen) HIP_TRY(hipStreamWaitEvent(stream, join_ev[0], 0));
+        if (fork && zsl) HIP_TRY(hipStreamWaitEvent(stream, join_ev[1], 0));
     }
 
     // Autotuner: time every tile shape on the (still all-zero) state and keep the fastest.

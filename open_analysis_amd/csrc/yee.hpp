@@ -173,23 +173,33 @@ __device__ __forceinline__ void yee_tile(const YeeArgs<R>& A, int tx, int ty, in
     const int hc_c = tid / (TYH + 1), hc_lj = tid - (tid / (TYH + 1)) * (TYH + 1);
     const int ih = i0 - 1 + TXH, jhc = j0 - 1 + hc_lj;
 
-    auto ld_e = [&](int c, int ii, int jj, int kk) -> R {
-        if (GEN && (ii >= X || jj >= Y || kk >= Z)) return (R)0;
-        return ldS<S, R>(ein[c], kk * sxy + jj * X + ii);
+    const S zs = static_cast<S>(0.0f);
+    auto ld_e = [&](int c, int ii, int jj, int kk) -> S {  // raw storage value, 0 outside
+        if (GEN && (ii >= X || jj >= Y || kk >= Z)) return zs;
+        return ldraw<S>(ein[c], kk * sxy + jj * X + ii);
     };
-    R e_pf[RPW][3], e_nx[RPW][3], e_own[RPW][3];
-    R row_pf[3], row_nx[3], col_pf = 0, col_nx = 0;
-    auto load_plane = [&](int kk, R (&e)[RPW][3], R (&row)[3], R& col) {
+    R e_nx[RPW][3], e_own[RPW][3], row_nx[3], col_nx = 0;
+    S e_pf[RPW][3], row_pf[3], col_pf = zs;  // E(k+2) in flight, in storage type
+    auto load_plane = [&](int kk, S (&e)[RPW][3], S (&row)[3], S& col) {
         const bool kin = kk < Z;
 #pragma unroll
         for (int r = 0; r < RPW; ++r)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) e[r][c] = kin ? ld_e(c, i, jr[r], kk) : (R)0;
+            for (int c = 0; c < 3; ++c) e[r][c] = kin ? ld_e(c, i, jr[r], kk) : zs;
         if (row_halo) {
 #pragma unroll
-            for (int c = 0; c < 3; ++c) row[c] = kin ? ld_e(c, i, jh, kk) : (R)0;
+            for (int c = 0; c < 3; ++c) row[c] = kin ? ld_e(c, i, jh, kk) : zs;
         }
-        if (col_halo) col = kin ? ld_e(hc_c, ih, jhc, kk) : (R)0;
+        if (col_halo) col = kin ? ld_e(hc_c, ih, jhc, kk) : zs;
+    };
+    auto widen = [&](const S (&e)[RPW][3], const S (&row)[3], S col, R (&eo)[RPW][3], R (&ro)[3], R& co) {
+#pragma unroll
+        for (int r = 0; r < RPW; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) eo[r][c] = cvt<S, R>(e[r][c]);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) ro[c] = cvt<S, R>(row[c]);
+        co = cvt<S, R>(col);
     };
     auto stage = [&](const R (&e)[RPW][3], const R (&row)[3], R col) {
 #pragma unroll
@@ -203,7 +213,8 @@ __device__ __forceinline__ void yee_tile(const YeeArgs<R>& A, int tx, int ty, in
         if (col_halo) sE[hc_c][hc_lj][TXH] = col;
     };
     // H^{n-1/2}, material rows and the single-pole Debye current of one plane
-    R hold_pf[RPW][3], jj_pf[RPW][3];
+    S hold_pf[RPW][3];
+    R jj_pf[RPW][3];
     int mat_pf[RPW][3];
     const bool ade1 = A.npole == 1;
     const long long jstride = (long long)A.njk * sxy;
@@ -220,7 +231,7 @@ __device__ __forceinline__ void yee_tile(const YeeArgs<R>& A, int tx, int ty, in
             const int jq = (kk - A.jk0) * sxy + jr[r] * X + i;
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                hold_pf[r][c] = ok ? ldS<S, R>(hin[c], q) : (R)0;
+                hold_pf[r][c] = ok ? ldraw<S>(hin[c], q) : zs;
                 mat_pf[r][c] = ecell ? (int)A.idx[c][q] : 0;
                 jj_pf[r][c] = (ecell && jin) ? Jb[c * (int)jstride + jq] : (R)0;
             }
@@ -229,11 +240,14 @@ __device__ __forceinline__ void yee_tile(const YeeArgs<R>& A, int tx, int ty, in
 
     // prologue: E(kstart) into LDS and registers; E(kstart+1), plane data of kstart in flight
     {
-        R r3[3], cl = 0;
-        load_plane(kstart, e_own, r3, cl);
-        stage(e_own, r3, cl);
+        S e3[RPW][3], r3[3], c3 = zs;
+        R r3w[3], clw = 0;
+        load_plane(kstart, e3, r3, c3);
+        widen(e3, r3, c3, e_own, r3w, clw);
+        stage(e_own, r3w, clw);
+        load_plane(kstart + 1, e3, r3, c3);
+        widen(e3, r3, c3, e_nx, row_nx, col_nx);
     }
-    load_plane(kstart + 1, e_nx, row_nx, col_nx);
     load_cells(kstart);
 
     R hprev[RPW][3], hcur[RPW][3];
@@ -250,7 +264,7 @@ __device__ __forceinline__ void yee_tile(const YeeArgs<R>& A, int tx, int ty, in
         for (int r = 0; r < RPW; ++r)
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                hold[r][c] = hold_pf[r][c];
+                hold[r][c] = cvt<S, R>(hold_pf[r][c]);
                 mat[r][c] = mat_pf[r][c];
                 jcur[r][c] = jj_pf[r][c];
             }
@@ -488,11 +502,8 @@ __device__ __forceinline__ void yee_tile(const YeeArgs<R>& A, int tx, int ty, in
             for (int c = 0; c < 3; ++c) {
                 hprev[r][c] = hcur[r][c];
                 e_own[r][c] = e_nx[r][c];
-                e_nx[r][c] = e_pf[r][c];
             }
-#pragma unroll
-        for (int c = 0; c < 3; ++c) row_nx[c] = row_pf[c];
-        col_nx = col_pf;
+        widen(e_pf, row_pf, col_pf, e_nx, row_nx, col_nx);
     }
 }
 
