@@ -6,6 +6,9 @@ Compiles ``oafdtd.hip`` into ``open_analysis_amd/csrc/liboafdtd_hip.so`` (the en
 ``ringdown.hip`` into ``liboaring_hip.so`` (the learned ringdown extrapolator) with hipcc,
 in-tree, so they travel with the source tree. Packed FP32 instructions are disabled
 (``-target-feature -packed-fp32-ops``); the compiler still vectorises ordinary float code.
+The engine is built with ``-ffp-contract=off``: every multiply and add is rounded on its own,
+so each tile shape and kernel (interior, z-slab, general, batched) gives the same bits and the
+autotuner's choice never changes a result.
 """
 
 from __future__ import annotations
@@ -21,6 +24,7 @@ ROOT = HERE.parents[1]
 LIB = HERE / "liboafdtd_hip.so"
 RING_LIB = HERE / "liboaring_hip.so"
 TARGETS = {LIB: HERE / "oafdtd.hip", RING_LIB: HERE / "ringdown.hip"}
+TARGET_FLAGS = {LIB: ["-ffp-contract=off"], RING_LIB: []}
 
 
 def hipcc() -> str:
@@ -46,7 +50,7 @@ def build(verbose: bool = False, extra: list[str] | None = None) -> Path:
     procs = []
     for lib, src in TARGETS.items():
         tmp = lib.with_suffix(".so.tmp")
-        cmd = _command(src, tmp, extra)
+        cmd = _command(src, tmp, TARGET_FLAGS[lib] + (extra or []))
         procs.append((lib, tmp, cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE,
                                                       stderr=subprocess.PIPE, text=True)))
     failed = []

@@ -21,7 +21,9 @@
 // and coefficient lookups hoisted out of the z loop), PMC mirror ghosts and the write masks of
 // the staggered lattice. Workgroups are mapped to tiles so that consecutive tiles (neighbours in
 // x, which share the cache lines their rows straddle, or in y, which share E halo rows; the
-// autotuner picks the order) run on the same XCD at the same time.
+// autotuner picks the order) run on the same XCD at the same time. The library is built with
+// -ffp-contract=off: every kernel and tile shape computes a cell (and a recomputed halo cell)
+// with the same operations and roundings, so results do not depend on the tile plan.
 //
 // Equations: packages/oa-fdtd/src/oa_fdtd/reference.py:152-266 (order :251-266) and
 // kernels/yee.cu:30-81, cpml :139-168; the Debye post-update (physics.cu:29-51) is folded into

@@ -135,6 +135,22 @@ def interior_block(k: Classes) -> Any:
     return m
 
 
+def tile_seam(k: Classes, nx: int, walls: tuple[str, ...]) -> Any:
+    """nx x 20 x 14 uniform 0.5 mm mesh with a 4-cell CPML where the walls ask for one; the
+    caller picks nx so that an x tile seam falls on boundary code. A Debye slab, two soft sources
+    off-centre and probes on both sides of the seams at x = 63 and 126."""
+    d = 0.5e-3
+    g = k["Grid"](np.arange(nx + 1) * d, np.arange(21) * d, np.arange(15) * d)
+    m = k["Model"](g, k["Waveform"]("gauss_mod", 12e9, 24e9), np.array([5e9, 15e9]), walls=walls)
+    m.cpml = k["CpmlSpec"](cells=4)
+    m.regions.append(k["Region"]((-1, -1, 4 * d), (1, 1, 8 * d), 3.6, poles=((0.4, 8e-12),)))
+    m.sources.append(k["SoftSource"]("s", 2, (100 * d, 9 * d, 6 * d), (100 * d, 9 * d, 6 * d)))
+    m.sources.append(k["SoftSource"]("t", 1, (40 * d, 5 * d, 10 * d), (40 * d, 5 * d, 10 * d)))
+    m.probes.append(k["Probe"]("e", "E", 2, (63 * d, 9 * d, 6 * d), (64 * d, 9 * d, 6 * d), "line"))
+    m.probes.append(k["Probe"]("h", "H", 1, (120 * d, 12 * d, 9 * d), (120 * d, 12 * d, 9 * d)))
+    return m
+
+
 CASES = {
     "gpu_cpu": (gpu_cpu_problem, 120),
     "debye_sibc_line": (debye_sibc_line, 150),

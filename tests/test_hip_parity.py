@@ -95,6 +95,43 @@ def test_fields_match_oracle_with_end_criterion() -> None:
     assert a.energy() == pytest.approx(b.energy(), rel=1e-4)
 
 
+# x tile t owns E columns [1 + 63 t, 64 + 63 t) and recomputes the H column 63 t below them
+SEAMS = {
+    # the recomputed H column 126 of the last tile inside the high x CPML slab (H 125..128); PMC y
+    # and z walls
+    "seam_cpml": (128, ("cpml", "cpml", "pmc", "pmc", "pmc", "pmc")),
+    # the x-high PMC wall (127) next to the recomputed column 126; y CPML, PEC z
+    "seam_pmc": (126, ("pmc", "pmc", "cpml", "cpml", "pec", "pec")),
+}
+
+
+@pytest.mark.parametrize("name", sorted(SEAMS))
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+def test_tile_seams_match_oracle(name: str, dtype: str, monkeypatch) -> None:
+    """Fields after 160 steps where x tile seams meet CPML slabs and PMC walls, for two tile
+    shapes (4x2 and 16x1), against the float64 oracle; both shapes give the same bits."""
+    import cases
+    from golden_io import OWN_CLASSES
+
+    nx, walls = SEAMS[name]
+    c = compile_model(cases.tile_seam(OWN_CLASSES, nx, walls))
+    ref = OracleEngine(c, "float64")
+    ref.step(160)
+    want = ref.get_fields()
+    got = []
+    for variant in (0, 6):
+        _say(f"[seam] case={name} dtype={dtype} variant={variant}")
+        monkeypatch.setenv("OAFD_YEE_VARIANT", str(variant))
+        eng = HipEngine(c, dtype, max_steps=200)
+        eng.step(160)
+        f = eng.get_fields()
+        err = _fields_err(f, want)
+        assert max(err.values()) < TOL[dtype], (variant, err)
+        got.append(f)
+    for k in got[0]:
+        assert np.array_equal(got[0][k], got[1][k]), k
+
+
 def test_graph_equals_eager() -> None:
     """Same device, same input: the 64-step graph and eager launches are bitwise identical
     (test_gpu_cpu.py:86-96)."""
