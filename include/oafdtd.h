@@ -18,6 +18,10 @@
  *   oafd_peak_bytes      CupyEngine.peak_bytes         engine.py:505-506
  *   oafd_estimate_bytes  estimate_bytes                engine.py:29-40
  *   oafd_sparams         source_spectrum / sparams     run.py:36-43, 203-246 (host in the reference)
+ *   oafd_gamma_from_planes  gamma_from_planes          ports/deembed.py:24-53, 85-101 (host NumPy)
+ *   oafd_smooth_gamma    smooth_gamma                  ports/deembed.py:55-82 (host NumPy)
+ *   oafd_shift_reference shift_reference               ports/deembed.py:104-121 (host NumPy)
+ *   oafd_wave_sparams    wave_sparams                  ports/deembed.py:264-338 (host NumPy)
  *   oafd_device_count/   oa_accel probe / select       accel/probe.py:92, accel/policy.py:147
  *   oafd_device_info
  *   oafd_debug_inject_fault  CupyBackend.inject_fault  accel/backend.py:217-219
@@ -135,6 +139,28 @@ int oafd_results(oafd_engine* e, int kind, double* out);
  * out: (n_ports, batch, n_freq, 2) = S[i, driven[k]]. */
 int oafd_sparams(oafd_engine* e, int n_ports, const int32_t* chan, const int32_t* pstart, const int32_t* pidx,
                  const double* r, const int32_t* driven, double* out);
+/* ---- plane-wave de-embedding in FP64 on the device (packages/oa-fdtd/src/oa_fdtd/ports/
+ * deembed.py, host NumPy in the reference). Complex arrays are (re, im) float64 pairs. These
+ * calls need no engine: each copies its inputs to the device, runs and copies the result back.
+ * gamma_from_planes (deembed.py:24-53): m rows of n_freq voltages at three planes (outer,
+ * middle, inner; spacing between planes 1-2, spacing_b between 2-3) -> the propagation
+ * constant with Re >= 0 and its phase unwrapped along frequency; out (m, n_freq). */
+int oafd_gamma_from_planes(int m, int n_freq, const double* v1, const double* v2, const double* v3, double spacing,
+                           double spacing_b, double* out);
+/* smooth_gamma (deembed.py:55-82), in place on (m, n_freq) rows; fit_mask (n_freq) or NULL */
+int oafd_smooth_gamma(int m, int n_freq, double* gamma, const double* freqs, double spacing, double min_phase,
+                      const uint8_t* fit_mask);
+/* shift_reference (deembed.py:104-121): s (n, n, n_freq), gamma (n, n_freq), lengths (n) */
+int oafd_shift_reference(int n, int n_freq, const double* s, const double* gamma, const double* lengths, double* out);
+/* wave_sparams (deembed.py:264-338) for n <= 16 ports: v (n, 3, n, n_freq) = the three plane
+ * voltages of port i when port j is driven; spacing, spacing_b, ref (middle plane to reference
+ * plane, positive inwards), port_r: (n); s_port (n, n, n_freq) or NULL; freqs (n_freq) or NULL
+ * (no smooth fit); fit_mask (n_freq) or NULL. Out: s (n, n, n_freq), gamma (n, n_freq) of
+ * excitation 0. */
+int oafd_wave_sparams(int n, int n_freq, const double* v, const double* spacing, const double* spacing_b,
+                      const double* ref, const double* port_r, const double* s_port, const double* freqs,
+                      const uint8_t* fit_mask, double min_phase, double floor_phase, double* s_out,
+                      double* gamma_out);
 /* steps recorded in the probe series */
 int64_t oafd_series_len(const oafd_engine* e);
 /* out: (steps, batch, n_probe[kind]) float64; cap = capacity in elements */

@@ -21,6 +21,7 @@
 
 #include "../../include/oafdtd.h"
 #include "aux_kernels.hpp"
+#include "deembed.hpp"
 #include "yee.hpp"
 
 namespace {
@@ -1464,6 +1465,30 @@ int check_desc(const oafd_desc* d, int dtype) {
 
 }  // namespace
 
+// Helpers of the de-embedding entry points.
+namespace {
+// device buffers of one call, freed when the call returns
+struct CallBufs {
+    std::vector<void*> p;
+    ~CallBufs() {
+        for (void* x : p) (void)hipFree(x);
+    }
+    template <typename T>
+    T* put(const T* host, size_t count) {
+        void* d = nullptr;
+        HIP_TRY(hipMalloc(&d, std::max<size_t>(count, 1) * sizeof(T)));
+        p.push_back(d);
+        if (host && count) HIP_TRY(hipMemcpy(d, host, count * sizeof(T), hipMemcpyHostToDevice));
+        return static_cast<T*>(d);
+    }
+};
+int blocks(long long n, int bs) { return (int)((n + bs - 1) / bs); }
+void sync_check() {
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipDeviceSynchronize());
+}
+}  // namespace
+
 extern "C" {
 
 const char* oafd_version(void) { return "0.1.0"; }
@@ -1643,6 +1668,118 @@ int oafd_copy_bandwidth(int64_t bytes, int reps, double* out) {
     if (a) (void)hipFree(a);
     if (b) (void)hipFree(b);
     return rc;
+}
+
+// ---- plane-wave de-embedding (FP64 on the device; packages/oa-fdtd/src/oa_fdtd/ports/deembed.py)
+
+int oafd_gamma_from_planes(int m, int nf, const double* v1, const double* v2, const double* v3, double spacing,
+                           double spacing_b, double* out) {
+    if (m < 1 || nf < 1 || !v1 || !v2 || !v3 || !out || !(spacing > 0.0) || !(spacing_b > 0.0))
+        return fail(OAFD_E_INVALID, "bad gamma_from_planes arguments");
+    if (oafd_device_count() < 1) return fail(OAFD_E_NO_DEVICE, "no ROCm device visible");
+    try {
+        CallBufs B;
+        const size_t n = (size_t)m * nf;
+        auto* d1 = B.put(reinterpret_cast<const oafd::cd*>(v1), n);
+        auto* d2 = B.put(reinterpret_cast<const oafd::cd*>(v2), n);
+        auto* d3 = B.put(reinterpret_cast<const oafd::cd*>(v3), n);
+        auto* g = B.put<oafd::cd>(nullptr, n);
+        hipLaunchKernelGGL(oafd::planes_gamma_k, dim3(blocks(m, 64)), dim3(64), 0, 0, m, nf, d1, d2, d3, spacing,
+                           spacing_b, g);
+        sync_check();
+        HIP_TRY(hipMemcpy(out, g, n * sizeof(oafd::cd), hipMemcpyDeviceToHost));
+    } catch (const HipFail& f) {
+        return fail(f.code, f.msg);
+    }
+    return OAFD_OK;
+}
+
+int oafd_smooth_gamma(int m, int nf, double* gamma, const double* freqs, double spacing, double min_phase,
+                      const uint8_t* fit_mask) {
+    if (m < 1 || nf < 1 || !gamma || !freqs || !(spacing > 0.0)) return fail(OAFD_E_INVALID, "bad smooth_gamma arguments");
+    if (oafd_device_count() < 1) return fail(OAFD_E_NO_DEVICE, "no ROCm device visible");
+    try {
+        CallBufs B;
+        const size_t n = (size_t)m * nf;
+        auto* g = B.put(reinterpret_cast<const oafd::cd*>(gamma), n);
+        auto* fq = B.put(freqs, (size_t)nf);
+        const unsigned char* mk = fit_mask ? B.put(reinterpret_cast<const unsigned char*>(fit_mask), (size_t)nf) : nullptr;
+        auto* scr = B.put<double>(nullptr, n * 9);
+        auto* use = B.put<unsigned char>(nullptr, n);
+        hipLaunchKernelGGL(oafd::smooth_gamma_k, dim3(blocks(m, 64)), dim3(64), 0, 0, m, nf, g, fq, spacing, min_phase,
+                           mk, scr, use);
+        sync_check();
+        HIP_TRY(hipMemcpy(gamma, g, n * sizeof(oafd::cd), hipMemcpyDeviceToHost));
+    } catch (const HipFail& f) {
+        return fail(f.code, f.msg);
+    }
+    return OAFD_OK;
+}
+
+int oafd_shift_reference(int n, int nf, const double* s, const double* gamma, const double* lengths, double* out) {
+    if (n < 1 || nf < 1 || !s || !gamma || !lengths || !out) return fail(OAFD_E_INVALID, "bad shift_reference arguments");
+    if (oafd_device_count() < 1) return fail(OAFD_E_NO_DEVICE, "no ROCm device visible");
+    try {
+        CallBufs B;
+        const size_t ns = (size_t)n * n * nf;
+        auto* ds = B.put(reinterpret_cast<const oafd::cd*>(s), ns);
+        auto* dg = B.put(reinterpret_cast<const oafd::cd*>(gamma), (size_t)n * nf);
+        auto* dl = B.put(lengths, (size_t)n);
+        auto* o = B.put<oafd::cd>(nullptr, ns);
+        hipLaunchKernelGGL(oafd::shift_reference_k, dim3(blocks((long long)ns, 256)), dim3(256), 0, 0, n, nf, ds, dg,
+                           dl, o);
+        sync_check();
+        HIP_TRY(hipMemcpy(out, o, ns * sizeof(oafd::cd), hipMemcpyDeviceToHost));
+    } catch (const HipFail& f) {
+        return fail(f.code, f.msg);
+    }
+    return OAFD_OK;
+}
+
+int oafd_wave_sparams(int n, int nf, const double* v, const double* spacing, const double* spacing_b,
+                      const double* ref, const double* port_r, const double* s_port, const double* freqs,
+                      const uint8_t* fit_mask, double min_phase, double floor_phase, double* s_out,
+                      double* gamma_out) {
+    if (n < 1 || n > oafd::MAXP || nf < 1 || !v || !spacing || !spacing_b || !ref || !port_r || !s_out || !gamma_out)
+        return fail(OAFD_E_INVALID, "bad wave_sparams arguments (1 to 16 ports)");
+    for (int i = 0; i < n; ++i)
+        if (!(spacing[i] > 0.0) || !(spacing_b[i] > 0.0) || !(port_r[i] > 0.0))
+            return fail(OAFD_E_INVALID, "plane spacings and port resistances must be positive");
+    if (oafd_device_count() < 1) return fail(OAFD_E_NO_DEVICE, "no ROCm device visible");
+    try {
+        CallBufs B;
+        const size_t nn = (size_t)n * n * nf;
+        oafd::WaveArgs W{};
+        W.n = n;
+        W.nf = nf;
+        W.v = B.put(reinterpret_cast<const oafd::cd*>(v), (size_t)n * 3 * n * nf);
+        W.da = B.put(spacing, (size_t)n);
+        W.db = B.put(spacing_b, (size_t)n);
+        W.ref = B.put(ref, (size_t)n);
+        W.port_r = B.put(port_r, (size_t)n);
+        W.freqs = freqs ? B.put(freqs, (size_t)nf) : nullptr;
+        W.mask = fit_mask ? B.put(reinterpret_cast<const unsigned char*>(fit_mask), (size_t)nf) : nullptr;
+        W.min_phase = min_phase;
+        W.floor_phase = floor_phase;
+        W.g = B.put<oafd::cd>(nullptr, nn);
+        W.a = B.put<oafd::cd>(nullptr, nn);
+        W.b = B.put<oafd::cd>(nullptr, nn);
+        W.scratch = freqs ? B.put<double>(nullptr, nn * 9) : nullptr;
+        W.use = freqs ? B.put<unsigned char>(nullptr, nn) : nullptr;
+        const oafd::cd* sp = s_port ? B.put(reinterpret_cast<const oafd::cd*>(s_port), nn) : nullptr;
+        auto* so = B.put<oafd::cd>(nullptr, nn);
+        hipLaunchKernelGGL(oafd::wave_rows_k, dim3(blocks(n * n, 64)), dim3(64), 0, 0, W);
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(oafd::wave_solve_k, dim3(blocks(nf, 64)), dim3(64), 0, 0, W, sp, so);
+        sync_check();
+        HIP_TRY(hipMemcpy(s_out, so, nn * sizeof(oafd::cd), hipMemcpyDeviceToHost));
+        for (int i = 0; i < n; ++i)  // gamma of port i under excitation 0
+            HIP_TRY(hipMemcpy(gamma_out + 2 * (size_t)i * nf, W.g + (size_t)i * n * nf, nf * sizeof(oafd::cd),
+                              hipMemcpyDeviceToHost));
+    } catch (const HipFail& f) {
+        return fail(f.code, f.msg);
+    }
+    return OAFD_OK;
 }
 
 int oafd_debug_inject_fault(oafd_engine* e, int code) {

@@ -75,6 +75,11 @@ def _declare(lib: C.CDLL) -> None:
         "oafd_plan": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, P, C.c_int, C.c_int, C.c_int, P,
                                 C.c_int, P, C.c_int, P]),
         "oafd_copy_bandwidth": (C.c_int, [C.c_int64, C.c_int, P]),
+        "oafd_gamma_from_planes": (C.c_int, [C.c_int, C.c_int, P, P, P, C.c_double, C.c_double, P]),
+        "oafd_smooth_gamma": (C.c_int, [C.c_int, C.c_int, P, P, C.c_double, C.c_double, P]),
+        "oafd_shift_reference": (C.c_int, [C.c_int, C.c_int, P, P, P, P]),
+        "oafd_wave_sparams": (C.c_int, [C.c_int, C.c_int, P, P, P, P, P, P, P, P, C.c_double,
+                                        C.c_double, P, P]),
         "oafd_debug_inject_fault": (C.c_int, [E, C.c_int]),
         "oafd_last_error": (C.c_char_p, [E]),
         "oafd_destroy": (None, [E]),
@@ -100,6 +105,7 @@ EXPORTS = (
     "oafd_energy", "oafd_results", "oafd_series_len", "oafd_sparams", "oafd_series", "oafd_get_field",
     "oafd_set_field", "oafd_step_count", "oafd_peak_bytes", "oafd_estimate_bytes",
     "oafd_step_timed", "oafd_describe", "oafd_plan", "oafd_copy_bandwidth",
+    "oafd_gamma_from_planes", "oafd_smooth_gamma", "oafd_shift_reference", "oafd_wave_sparams",
     "oafd_debug_inject_fault", "oafd_last_error", "oafd_destroy",
     "oafd_batch_create", "oafd_batch_step", "oafd_batch_step_timed", "oafd_batch_record",
     "oafd_batch_count",

@@ -86,3 +86,48 @@ def test_job_runs_on_the_hip_engine(tmp_path: Path, capsys) -> None:
     _, s_ts, z0, names = read_touchstone((tmp_path / "ports.s2p").read_text())
     assert names == ["p1", "p2"]
     assert np.allclose(s_ts, np.moveaxis(s, 2, 0), rtol=1e-11, atol=1e-15)
+
+
+def test_worker_without_ports_exits_2(tmp_path: Path, capsys) -> None:
+    from open_analysis_amd.fdtd import worker
+
+    p = _job(tmp_path)
+    job = json.loads(p.read_text())
+    job["ports"] = []
+    p.write_text(json.dumps(job))
+    assert worker.main([str(p)]) == 2
+    err = json.loads(capsys.readouterr().out.strip().splitlines()[-1])
+    assert err["code"] == "FDTD_BAD_JOB" and err["path"] == str(p)
+
+
+def test_worker_cpu_engine_request_exits_2(tmp_path: Path, capsys) -> None:
+    from open_analysis_amd.fdtd import worker
+
+    assert worker.main([str(_job(tmp_path, engine="numpy"))]) == 2
+    assert json.loads(capsys.readouterr().out.strip().splitlines()[-1])["code"] == "ACCEL_NO_CPU_ENGINE"
+
+
+@pytest.mark.gpu
+def test_worker_writes_ports_npz_and_run_json(tmp_path: Path, capsys) -> None:
+    """The adapter worker's files (worker.py:47-51): ports.npz with s as (F, N, N), z0 and names,
+    and run.json with the run_meta fields; S equal to run() + sparams() on the same job."""
+    from open_analysis_amd.fdtd import run, sparams, worker
+
+    p = _job(tmp_path)
+    assert worker.main([str(p)]) == 0
+    lines = capsys.readouterr().out.strip().splitlines()
+    assert lines[-2] == "step 400"
+    assert json.loads(lines[-1]) == {"steps": 400, "stopped_by": "max_steps", "engine": "hip"}
+    out = np.load(tmp_path / "ports.npz")
+    assert list(out["names"]) == ["p1", "p2"] and np.array_equal(out["z0"], [188.4, 188.4])
+    model = cli.model_from_job(json.loads(p.read_text()))
+    res = run(model, engine="hip", dtype="float32", max_steps=400, end_db=None)
+    assert out["s"].shape == (6, 2, 2)
+    assert np.array_equal(out["s"], np.moveaxis(sparams(model, res), 2, 0))
+    assert np.array_equal(out["freqs"], model.frequencies)
+    meta = json.loads((tmp_path / "run.json").read_text())
+    for k in ("engine", "dtype", "steps", "stopped_by", "energy_db", "dt_s", "peak_gpu_bytes",
+              "wall_build_s", "wall_run_s", "cells", "diagnostics", "accel", "tail",
+              "fdtd_version", "port_names"):
+        assert k in meta, k
+    assert meta["cells"] == 4 * 40 * 4 and meta["port_names"] == ["p1", "p2"]
