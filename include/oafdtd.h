@@ -18,6 +18,9 @@
  *   oafd_peak_bytes      CupyEngine.peak_bytes         engine.py:505-506
  *   oafd_estimate_bytes  estimate_bytes                engine.py:29-40
  *   oafd_sparams         source_spectrum / sparams     run.py:36-43, 203-246 (host in the reference)
+ *   oafd_compile_edge_average     _edge_average     model.py:300-321 (host NumPy in the reference)
+ *   oafd_compile_material_table   material table    model.py:688-722 (np.unique; host NumPy)
+ *   oafd_compile_ade_edges        ADE edge lists    model.py:331-367 (host NumPy)
  *   oafd_gamma_from_planes  gamma_from_planes          ports/deembed.py:24-53, 85-101 (host NumPy)
  *   oafd_smooth_gamma    smooth_gamma                  ports/deembed.py:55-82 (host NumPy)
  *   oafd_shift_reference shift_reference               ports/deembed.py:104-121 (host NumPy)
@@ -139,6 +142,28 @@ int oafd_results(oafd_engine* e, int kind, double* out);
  * out: (n_ports, batch, n_freq, 2) = S[i, driven[k]]. */
 int oafd_sparams(oafd_engine* e, int n_ports, const int32_t* chan, const int32_t* pstart, const int32_t* pidx,
                  const double* r, const int32_t* driven, double* out);
+/* ---- native core of compile_model (host C++, multi-threaded, no device needed). Each result
+ * has the bits of the NumPy formulation of the same step (same operations in the same order).
+ * n = (nx, ny, nz). Lattice arrays are (nz+1, ny+1, nx+1) float64 / uint8; cell arrays
+ * (nz+2, ny+2, nx+2); storage arrays (nz+2, ny+2, pitch).
+ * edge_average (model.py:300-321): m cell arrays -> m lattice arrays for edges of component comp,
+ * the area-weighted mean over the up to four cells around each edge; wx, wy, wz = cell weights
+ * (primal lengths at I = i + 1, zero on ghosts), lengths nx+2, ny+2, nz+2. */
+int oafd_compile_edge_average(const int32_t* n, int comp, const double* wx, const double* wy, const double* wz, int m,
+                              const double* const* cells, double* const* out);
+/* material table (model.py:688-722): sorted unique (round(eps, 12), round(sigma, 12)) over the
+ * non-PEC edges of the three components -> tab_eps/tab_sig (capacity max_rows), *n_rows (-1
+ * when there are more than max_rows), and the three uint16 storage index arrays (row + 1,
+ * 0 = PEC). */
+int oafd_compile_material_table(const int32_t* n, int pitch, const double* const* eps, const double* const* sig,
+                                const uint8_t* const* pec, int max_rows, double* tab_eps, double* tab_sig,
+                                uint16_t* const* idx, int64_t* n_rows);
+/* Debye edge list of one component (model.py:331-367): with flat == NULL, *ne = the number of
+ * non-PEC edges whose pole strengths sum above 1e-12; otherwise (cap == that count) fills, in C
+ * order, flat (storage index), beta (n_poles, ne) = eps0 del / (tau + dt/2), cbe = cb_tab[idx]. */
+int oafd_compile_ade_edges(const int32_t* n, int pitch, int n_poles, const double* const* del, const uint8_t* pec,
+                           const double* taus, double dt, double eps0, const uint16_t* idx, const double* cb_tab,
+                           int64_t cap, int64_t* ne, int64_t* flat, double* beta, double* cbe);
 /* ---- plane-wave de-embedding in FP64 on the device (packages/oa-fdtd/src/oa_fdtd/ports/
  * deembed.py, host NumPy in the reference). Complex arrays are (re, im) float64 pairs. These
  * calls need no engine: each copies its inputs to the device, runs and copies the result back.

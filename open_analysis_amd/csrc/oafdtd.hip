@@ -21,6 +21,7 @@
 
 #include "../../include/oafdtd.h"
 #include "aux_kernels.hpp"
+#include "compile_core.hpp"
 #include "deembed.hpp"
 #include "yee.hpp"
 
@@ -1778,6 +1779,69 @@ int oafd_wave_sparams(int n, int nf, const double* v, const double* spacing, con
                               hipMemcpyDeviceToHost));
     } catch (const HipFail& f) {
         return fail(f.code, f.msg);
+    }
+    return OAFD_OK;
+}
+
+// ---- native core of compile_model (host only; packages/oa-fdtd/src/oa_fdtd/model.py)
+int oafd_compile_edge_average(const int32_t* n, int comp, const double* wx, const double* wy, const double* wz, int m,
+                              const double* const* cells, double* const* out) {
+    if (!n || n[0] < 1 || n[1] < 1 || n[2] < 1 || comp < 0 || comp > 2 || !wx || !wy || !wz || m < 1 || !cells || !out)
+        return fail(OAFD_E_INVALID, "bad edge_average arguments");
+    const int nn[3] = {n[0], n[1], n[2]};
+    const double* w[3] = {wx, wy, wz};
+    try {
+        oafd_compile::edge_average(nn, comp, w, m, cells, out);
+    } catch (const std::exception& e) {
+        return fail(OAFD_E_OOM, std::string("edge_average: ") + e.what());
+    }
+    return OAFD_OK;
+}
+
+int oafd_compile_material_table(const int32_t* n, int pitch, const double* const* eps, const double* const* sig,
+                                const uint8_t* const* pec, int max_rows, double* tab_eps, double* tab_sig,
+                                uint16_t* const* idx, int64_t* n_rows) {
+    if (!n || n[0] < 1 || n[1] < 1 || n[2] < 1 || pitch < n[0] + 2 || !eps || !sig || !pec || max_rows < 1 ||
+        max_rows > 65535 || !tab_eps || !tab_sig || !idx || !n_rows)
+        return fail(OAFD_E_INVALID, "bad material_table arguments");
+    const int nn[3] = {n[0], n[1], n[2]};
+    const double* e3[3] = {eps[0], eps[1], eps[2]};
+    const double* s3[3] = {sig[0], sig[1], sig[2]};
+    const uint8_t* p3[3] = {pec[0], pec[1], pec[2]};
+    uint16_t* i3[3] = {idx[0], idx[1], idx[2]};
+    try {
+        std::vector<oafd_compile::Key> table;
+        const long long rows = oafd_compile::material_table(nn, pitch, e3, s3, p3, max_rows, table, i3);
+        *n_rows = rows;
+        if (rows < 0) return OAFD_OK;  // more distinct materials than max_rows: the caller reports it
+        for (long long r = 0; r < rows; ++r) {
+            tab_eps[r] = table[r].e;
+            tab_sig[r] = table[r].s;
+        }
+    } catch (const std::exception& e) {
+        return fail(OAFD_E_OOM, std::string("material_table: ") + e.what());
+    }
+    return OAFD_OK;
+}
+
+int oafd_compile_ade_edges(const int32_t* n, int pitch, int n_poles, const double* const* del, const uint8_t* pec,
+                           const double* taus, double dt, double eps0, const uint16_t* idx, const double* cb_tab,
+                           int64_t cap, int64_t* ne, int64_t* flat, double* beta, double* cbe) {
+    if (!n || n[0] < 1 || n[1] < 1 || n[2] < 1 || pitch < n[0] + 2 || n_poles < 1 || !del || !pec || !ne)
+        return fail(OAFD_E_INVALID, "bad ade_edges arguments");
+    const int nn[3] = {n[0], n[1], n[2]};
+    try {
+        if (!flat) {  // count pass
+            *ne = oafd_compile::ade_edges(nn, pitch, n_poles, del, pec, taus, dt, eps0, idx, cb_tab, nullptr);
+            return OAFD_OK;
+        }
+        if (!taus || !idx || !cb_tab || !beta || !cbe) return fail(OAFD_E_INVALID, "bad ade_edges outputs");
+        const long long want = oafd_compile::ade_edges(nn, pitch, n_poles, del, pec, taus, dt, eps0, idx, cb_tab, nullptr);
+        if (want != cap) return fail(OAFD_E_INVALID, "ade_edges capacity does not match the count");
+        const oafd_compile::AdeOut o{flat, beta, cbe};
+        *ne = oafd_compile::ade_edges(nn, pitch, n_poles, del, pec, taus, dt, eps0, idx, cb_tab, &o);
+    } catch (const std::exception& e) {
+        return fail(OAFD_E_OOM, std::string("ade_edges: ") + e.what());
     }
     return OAFD_OK;
 }

@@ -75,6 +75,10 @@ def _declare(lib: C.CDLL) -> None:
         "oafd_plan": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, P, C.c_int, C.c_int, C.c_int, P,
                                 C.c_int, P, C.c_int, P]),
         "oafd_copy_bandwidth": (C.c_int, [C.c_int64, C.c_int, P]),
+        "oafd_compile_edge_average": (C.c_int, [P, C.c_int, P, P, P, C.c_int, P, P]),
+        "oafd_compile_material_table": (C.c_int, [P, C.c_int, P, P, P, C.c_int, P, P, P, P]),
+        "oafd_compile_ade_edges": (C.c_int, [P, C.c_int, C.c_int, P, P, P, C.c_double, C.c_double, P, P,
+                                             C.c_int64, P, P, P, P]),
         "oafd_gamma_from_planes": (C.c_int, [C.c_int, C.c_int, P, P, P, C.c_double, C.c_double, P]),
         "oafd_smooth_gamma": (C.c_int, [C.c_int, C.c_int, P, P, C.c_double, C.c_double, P]),
         "oafd_shift_reference": (C.c_int, [C.c_int, C.c_int, P, P, P, P]),
@@ -106,6 +110,7 @@ EXPORTS = (
     "oafd_set_field", "oafd_step_count", "oafd_peak_bytes", "oafd_estimate_bytes",
     "oafd_step_timed", "oafd_describe", "oafd_plan", "oafd_copy_bandwidth",
     "oafd_gamma_from_planes", "oafd_smooth_gamma", "oafd_shift_reference", "oafd_wave_sparams",
+    "oafd_compile_edge_average", "oafd_compile_material_table", "oafd_compile_ade_edges",
     "oafd_debug_inject_fault", "oafd_last_error", "oafd_destroy",
     "oafd_batch_create", "oafd_batch_step", "oafd_batch_step_timed", "oafd_batch_record",
     "oafd_batch_count",

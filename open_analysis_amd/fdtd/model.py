@@ -288,32 +288,28 @@ def _cell_arrays(model: Model, taus: FloatArray) -> tuple[FloatArray, FloatArray
     return eps, sig, dels
 
 
+def _ptrs(arrays: list[np.ndarray]) -> Any:
+    import ctypes as C
+
+    return (C.c_void_p * len(arrays))(*[a.ctypes.data for a in arrays])
+
+
 def _edge_average(g: Grid, comp: int, arrays: list[FloatArray]) -> list[FloatArray]:
     """Area-weighted mean of the (up to) four cells around every ``comp`` edge, on the
-    ``(nz+1, ny+1, nx+1)`` lattice ``[k-1, j-1, i-1]`` (model.py:300-321). The weights are the
-    primal lengths of the two transverse axes; ghost cells weigh zero."""
+    ``(nz+1, ny+1, nx+1)`` lattice ``[k-1, j-1, i-1]`` (model.py:300-321); the weights are the
+    primal lengths of the two transverse axes, ghost cells weigh zero. Native
+    (``oafd_compile_edge_average``), with the bits of the NumPy formulation."""
+    from . import _lib
+
     n = g.n
-    p, q = [a for a in range(3) if a != comp]
-    wts = [g.cell_weights(a)[: n[a] + 2] for a in range(3)]
-    out_shape = (n[2] + 1, n[1] + 1, n[0] + 1)
-    num = [np.zeros(out_shape) for _ in arrays]
-    den = np.zeros(out_shape)
-    for dp in (0, 1):
-        for dq in (0, 1):
-            shift = [0, 0, 0]
-            shift[p], shift[q] = dp, dq
-            sl = tuple(slice(1 - shift[a], n[a] + 2 - shift[a]) for a in (2, 1, 0))
-            wp = wts[p][1 - dp : n[p] + 2 - dp]
-            wq = wts[q][1 - dq : n[q] + 2 - dq]
-            shp_p = [1, 1, 1]
-            shp_p[2 - p] = wp.size
-            shp_q = [1, 1, 1]
-            shp_q[2 - q] = wq.size
-            w = wp.reshape(shp_p) * wq.reshape(shp_q)
-            for acc, arr in zip(num, arrays, strict=True):
-                acc += w * arr[sl]
-            den += w
-    return [a / den for a in num]
+    wts = [np.ascontiguousarray(g.cell_weights(a)[: n[a] + 2]) for a in range(3)]
+    cells = [np.ascontiguousarray(a, dtype=np.float64) for a in arrays]
+    out = [np.empty((n[2] + 1, n[1] + 1, n[0] + 1)) for _ in arrays]
+    nn = np.array(n, np.int32)
+    _lib.check(_lib.lib().oafd_compile_edge_average(
+        nn.ctypes.data, comp, wts[0].ctypes.data, wts[1].ctypes.data, wts[2].ctypes.data, len(cells),
+        _ptrs(cells), _ptrs(out)), "compile_model")
+    return out
 
 
 def _compile_ade(
@@ -325,24 +321,37 @@ def _compile_ade(
     idx: list[npt.NDArray[np.uint16]],
     cb_tab: FloatArray,
 ) -> AdeData | None:
-    """Dispersive edge lists: any pole strength > 1e-12 on a non-PEC edge (model.py:331-367)."""
+    """Dispersive edge lists: any pole strength > 1e-12 on a non-PEC edge (model.py:331-367).
+    Native (``oafd_compile_ade_edges``), C order, with the bits of the NumPy formulation."""
     if taus.size == 0:
         return None
+    import ctypes as C
+
+    from . import _lib
+
+    lib = _lib.lib()
+    nn = np.array(g.n, np.int32)
+    tau = np.ascontiguousarray(taus, np.float64)
+    cbt = np.ascontiguousarray(cb_tab, np.float64)
     comps, flats, betas, cbes = [], [], [], []
     for c in range(3):
-        total = np.zeros_like(edge_del[c][0])
-        for d in edge_del[c]:
-            total += d
-        mask = (total > 1e-12) & ~pec[c]
-        k, j, i = np.nonzero(mask)
-        if k.size == 0:
+        dels = [np.ascontiguousarray(d, np.float64) for d in edge_del[c]]
+        pc = np.ascontiguousarray(pec[c]).view(np.uint8)
+        ne = C.c_int64(0)
+        args = (nn.ctypes.data, g.pitch, taus.size, _ptrs(dels), pc.ctypes.data, tau.ctypes.data, float(dt),
+                float(EPS0), idx[c].ctypes.data, cbt.ctypes.data)
+        _lib.check(lib.oafd_compile_ade_edges(*args, 0, C.byref(ne), None, None, None), "compile_model")
+        if ne.value == 0:
             continue
-        zs, ys, xs = g.shape
-        flats.append((((k + 1) * ys + (j + 1)) * xs + (i + 1)).astype(np.int64))
-        betas.append(np.stack([EPS0 * edge_del[c][pi][mask] / (taus[pi] + 0.5 * dt)
-                               for pi in range(taus.size)]))
-        cbes.append(cb_tab[idx[c][k + 1, j + 1, i + 1]])
+        flat = np.empty(ne.value, np.int64)
+        beta = np.empty((taus.size, ne.value))
+        cbe = np.empty(ne.value)
+        _lib.check(lib.oafd_compile_ade_edges(*args, ne.value, C.byref(ne), flat.ctypes.data, beta.ctypes.data,
+                                              cbe.ctypes.data), "compile_model")
         comps.append(c)
+        flats.append(flat)
+        betas.append(beta)
+        cbes.append(cbe)
     if not comps:
         return None
     h = 0.5 * dt / taus
@@ -439,26 +448,32 @@ def _compile_conductors(
     )
 
 
-def _material_table(
-    edge_eps: list[FloatArray], edge_sig: list[FloatArray], pec: list[npt.NDArray[np.bool_]]
-) -> tuple[FloatArray, FloatArray, list[npt.NDArray[np.intp]]]:
+def _material_index(
+    g: Grid, edge_eps: list[FloatArray], edge_sig: list[FloatArray], pec: list[npt.NDArray[np.bool_]]
+) -> tuple[FloatArray, FloatArray, list[npt.NDArray[np.uint16]]]:
     """Sorted unique ``(round(eps, 12), round(sigma, 12))`` pairs over the non-PEC edges of all
-    components (model.py:688-722); returns the table and per-component row indices (1-based)."""
-    per_u, per_inv = [], []
-    for c in range(3):
-        keep = ~pec[c]
-        key = np.round(edge_eps[c][keep], 12) + 1j * np.round(edge_sig[c][keep], 12)
-        u, inv = np.unique(key, return_inverse=True)
-        per_u.append(u)
-        per_inv.append(inv.ravel())
-    uni, merge = np.unique(np.concatenate(per_u), return_inverse=True)
-    merge = merge.ravel()
-    rows = []
-    off = 0
-    for u, inv in zip(per_u, per_inv, strict=True):
-        rows.append(merge[off : off + u.size][inv] + 1)
-        off += u.size
-    return uni.real.copy(), uni.imag.copy(), rows
+    components and the uint16 storage index arrays (row + 1, 0 = PEC) (model.py:688-722).
+    Native (``oafd_compile_material_table``): the table np.unique gives on the complex keys."""
+    import ctypes as C
+
+    from . import _lib
+
+    nn = np.array(g.n, np.int32)
+    e = [np.ascontiguousarray(a, np.float64) for a in edge_eps]
+    s_ = [np.ascontiguousarray(a, np.float64) for a in edge_sig]
+    p = [np.ascontiguousarray(a).view(np.uint8) for a in pec]
+    idx = [np.empty(g.shape, dtype=np.uint16) for _ in range(3)]
+    cap = MAX_TABLE - 1
+    te, ts = np.empty(cap), np.empty(cap)
+    rows = C.c_int64(0)
+    _lib.check(_lib.lib().oafd_compile_material_table(
+        nn.ctypes.data, g.pitch, _ptrs(e), _ptrs(s_), _ptrs(p), cap, te.ctypes.data, ts.ctypes.data,
+        _ptrs(idx), C.byref(rows)), "compile_model")
+    if rows.value < 0:
+        raise ModelError("FDTD_TOO_MANY_MATERIALS",
+                         f"more than {cap} averaged materials exceed the uint16 table",
+                         "Use fewer distinct materials or a more uniform grid.")
+    return te[: rows.value].copy(), ts[: rows.value].copy(), idx
 
 
 def compile_model(model: Model, dtype: str = "float32") -> Compiled:
@@ -579,20 +594,9 @@ def compile_model(model: Model, dtype: str = "float32") -> Compiled:
         probes.append({"name": pr.name, "kind": pr.kind, "comp": pr.axis, "flat": flat,
                        "weights": w})
 
-    eps_u, sig_u, rows = _material_table(edge_eps, edge_sig, pec)
-    if eps_u.size + 1 > MAX_TABLE:
-        raise ModelError("FDTD_TOO_MANY_MATERIALS",
-                         f"{eps_u.size} averaged materials exceed the uint16 table",
-                         "Use fewer distinct materials or a more uniform grid.")
+    eps_u, sig_u, idx = _material_index(g, edge_eps, edge_sig, pec)
     eps_tab = np.concatenate(([0.0], eps_u))
     sig_tab = np.concatenate(([0.0], sig_u))
-    idx: list[npt.NDArray[np.uint16]] = []
-    for c in range(3):
-        lattice = np.zeros(lat, dtype=np.uint16)
-        lattice[~pec[c]] = rows[c].astype(np.uint16)
-        arr = np.zeros(g.shape, dtype=np.uint16)
-        arr[1 : nz + 2, 1 : ny + 2, 1 : nx + 2] = lattice
-        idx.append(arr)
     ep = eps_tab * EPS0
     ep_safe = np.where(ep > 0, ep, 1.0)
     loss = sig_tab * dt / (2.0 * ep_safe)
