@@ -1,7 +1,7 @@
 """CPU oracle for the oa-fdtd time-stepping path.
 
 TEST INFRASTRUCTURE: imported only by ``tests/``, ``__graft_entry__.smoke()`` and the
-CPU-baseline leg of ``bench.py``, as the checker. The product package never imports it.
+CPU-baseline leg of ``bench.py``, to compare against. The product package never imports it.
 """
 
 from .numpy_engine import OracleEngine, oracle_factory

@@ -1,7 +1,7 @@
 """CPU oracle of the oa-fdtd time step: a NumPy restatement of the reference engine.
 
 TEST INFRASTRUCTURE ONLY. Nothing in the product package imports this module; only ``tests/``,
-``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline leg use it, as the checker.
+``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline leg use it, to compare against.
 
 It restates ``packages/oa-fdtd/src/oa_fdtd/reference.py:41-331`` (``NumpyEngine``): the same
 equations as vectorised slices and the same step order (``reference.py:251-266``):
