@@ -57,6 +57,20 @@ def test_batch_members_match_single_runs_bitwise(small_set) -> None:
         assert batch.members[i].energy() == eng.energy()
 
 
+@pytest.mark.parametrize("dtype", ["float16", "bfloat16", "float64"])
+def test_batch_members_match_single_runs_in_every_storage_form(small_set, dtype: str) -> None:
+    """16-bit storage (with the FP16 field scale) and FP64: batch members equal their own
+    engines bit for bit, as in FP32."""
+    _, cs = small_set
+    batch = HipBatch(cs[:2], dtype, max_steps=STEPS)
+    batch.step(STEPS)
+    for i, c in enumerate(cs[:2]):
+        _say(f"[batch] {dtype} member {i} vs its own engine")
+        eng = HipEngine(c, dtype, max_steps=STEPS)
+        eng.step(STEPS)
+        _assert_same(_state(batch.members[i]), _state(eng))
+
+
 def test_subset_batch_is_invariant(small_set) -> None:
     _, cs = small_set
     full = HipBatch(cs, "float32", max_steps=STEPS)
