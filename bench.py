@@ -9,7 +9,9 @@ with every input resident in HBM before the timed region. ``roofline`` is for th
 kernel ``yee_step``: its algorithmic bytes per launch (six field components read and written,
 the 3 x uint16 material index, the Debye current and the CPML psi of every cell that has them)
 divided by its mean launch time from HIP events on the engine's stream. ``cpu_baseline`` is the
-float64 NumPy oracle stepping the same compiled problem on one host core.
+float64 NumPy oracle stepping the same compiled problem on one host core. ``time_to_s21`` is the
+metric's second half: time to S21 within 0.05 dB, the learned ringdown extrapolator with FP16
+storage against the plain FP32 engine, on held-out via transitions (``--s21-geometries``).
 """
 
 from __future__ import annotations
@@ -164,6 +166,53 @@ def bench_batch(a) -> dict:
     return out
 
 
+RINGNET = ROOT / "open_analysis_amd" / "ringdown" / "via_ringnet.npz"
+
+
+def time_to_s21(n_geo: int) -> dict | None:
+    """The second half of the metric: time to S21 within 0.05 dB over 1-20 GHz, learned plus
+    reduced precision against plain FP32, on ``n_geo`` held-out seeded via transitions (4 ports,
+    each driven once). Plain: the FP32 engine to the full 15000 steps. Learned: FP16 storage for
+    K + W steps, the trained network's pole-residue tail from K + W on, the residual-energy check
+    over [K, K + W) and, for the rows that fail it, the full run (its time counted pro rata).
+    Both are judged against FP64 full-length runs of the same grids. Times are GPU wall times
+    of the batched runs (one batch per form) plus the network's inference."""
+    if not RINGNET.exists():
+        return None
+    from open_analysis_amd.ringdown import extrapolate as X
+    from open_analysis_amd.ringdown import study as RS
+
+    net, norm, meta = RS.load_network(str(RINGNET))
+    K, W, thr, n_full = meta["K"], meta["W"], meta["threshold"], RS.N_DEFAULT
+    models, desc = RS.geometry(n_geo, 2000)  # the study's held-out seed; training used 1000
+    say = lambda m: print(f"[bench] time-to-S21: {m}", file=sys.stderr, flush=True)  # noqa: E731
+    rs32 = X.simulate(models, desc, K, W, n_full, dtype="float32")
+    say(f"plain FP32 {n_full} steps {rs32.wall_s['fdtd']:.1f} s")
+    rs16 = X.simulate(models, desc, K, W, K + W, dtype="float16")
+    say(f"FP16 {K + W} steps {rs16.wall_s['fdtd']:.1f} s")
+    rs64 = X.simulate(models, desc, K, W, n_full, dtype="float64")
+    say(f"FP64 reference {rs64.wall_s['fdtd']:.1f} s")
+    rs16.dft_full = rs32.dft_full  # a failed check falls back to the full run
+    idx = np.arange(len(rs16.driven))
+    t0 = time.perf_counter()
+    pred = X.predict(net, rs16, idx, norm, thr)
+    t_pred = time.perf_counter() - t0
+    fb = float(pred.fallback.mean())
+    plain_s = rs32.wall_s["fdtd"]
+    learned_s = rs16.wall_s["fdtd"] + t_pred + fb * plain_s
+    learned = RS.judge(rs16, idx, pred.s_params, rs64.dft_full)["through"]
+    plain = RS.judge(rs32, idx, rs32.dft_full, rs64.dft_full)["through"]
+    return {
+        "workload": "held-out seeded via transitions (4 ports, 1-20 GHz)", "geometries": n_geo,
+        "rows": int(len(idx)), "K": K, "W": W, "full_steps": n_full, "threshold": thr,
+        "network": {"file": RINGNET.name, "train_geometries": meta["train_geometries"]},
+        "plain_fp32_s": plain_s, "learned_fp16_s": learned_s, "inference_s": t_pred,
+        "speedup": plain_s / learned_s, "fallback_rate": fb,
+        "s21_err_db_learned_fp16": learned, "s21_err_db_plain_fp32": plain,
+        "target_db": 0.05, "within_target": bool(learned["max_db"] <= 0.05),
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -174,6 +223,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--geometries", type=int, default=32, help="via_batch: number of geometries")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--s21-geometries", type=int, default=16,
+                    help="held-out via geometries of the time-to-S21 leg (0 skips it)")
     ap.add_argument("--calibrate-copy", type=float, default=0.0, metavar="GIB",
                     help="only run the plain copy kernel over GIB gibibytes and print its rate")
     a = ap.parse_args()
@@ -234,6 +285,13 @@ def main() -> int:
         "tiling": tiling,
     }
     del eng
+    if a.s21_geometries > 0:
+        try:
+            t2 = time_to_s21(a.s21_geometries)
+        except Exception as exc:  # the throughput line above stands on its own
+            t2 = {"error": f"{type(exc).__name__}: {exc}"}
+        if t2 is not None:
+            out["time_to_s21"] = t2
     if not a.no_cpu:
         print("[bench] timing the CPU oracle", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(c, a.cpu_seconds)

@@ -64,10 +64,29 @@ def pulse_end(model) -> int:
     return int(np.ceil(2.0 * wf.delay_tau * wf.tau / compile_model(model).dt))
 
 
+def save_network(path: str, params: np.ndarray, norm: X.Normaliser, K: int, W: int, threshold: float,
+                 n_train: int, seed: int = 0) -> None:
+    """Write a trained extrapolator: parameters, descriptor normaliser and the run constants."""
+    np.savez_compressed(path, params=params.astype(np.float32), norm_mean=norm.mean, norm_std=norm.std,
+                        K=K, W=W, threshold=threshold, freqs=FREQS, train_geometries=n_train, seed=seed)
+
+
+def load_network(path: str, batch: int = 64) -> tuple[RingNet, X.Normaliser, dict[str, Any]]:
+    """A trained extrapolator written by :func:`save_network` (numpy arrays only; no pickle)."""
+    with np.load(path, allow_pickle=False) as z:
+        meta = {k: z[k] for k in ("K", "W", "threshold", "train_geometries", "seed")}
+        meta = {k: float(v) if k == "threshold" else int(v) for k, v in meta.items()}
+        freqs = np.array(z["freqs"])
+        net = RingNet(batch, X.time_len(meta["K"]), freqs.size, params=np.array(z["params"]))
+        norm = X.Normaliser(np.array(z["norm_mean"]), np.array(z["norm_std"]))
+    meta["freqs"] = freqs
+    return net, norm, meta
+
+
 def run_study(n_train: int, n_test: int, epochs: int, batch: int = 64, lr: float = 2e-3,
               K: int = K_DEFAULT, W: int = W_DEFAULT, n_full: int = N_DEFAULT,
               threshold: float = 0.25, fp16: bool = False, seed: int = 0, log: Any = None,
-              params: np.ndarray | None = None) -> tuple[dict[str, Any], np.ndarray]:
+              params: np.ndarray | None = None) -> tuple[dict[str, Any], np.ndarray, X.Normaliser]:
     t_all = time.perf_counter()
     m_tr, d_tr = geometry(n_train, 1000 + seed)
     m_te, d_te = geometry(n_test, 2000 + seed)
@@ -114,7 +133,7 @@ def run_study(n_train: int, n_test: int, epochs: int, batch: int = 64, lr: float
         report["simulate_s"]["fp16"] = rs16.wall_s
         report["simulate_s"]["fp64"] = rs64.wall_s
     report["total_s"] = time.perf_counter() - t_all
-    return report, net.params()
+    return report, net.params(), norm
 
 
 def main(argv: list[str] | None = None) -> int:
@@ -126,16 +145,17 @@ def main(argv: list[str] | None = None) -> int:
     ap.add_argument("--threshold", type=float, default=0.25)
     ap.add_argument("--fp16", action="store_true")
     ap.add_argument("--out", default=None)
-    ap.add_argument("--params", default=None, help="write the trained parameters (npz)")
+    ap.add_argument("--params", default=None,
+                    help="write the trained network, normaliser and constants (npz, see load_network)")
     a = ap.parse_args(argv)
-    rep, p = run_study(a.train, a.test, a.epochs, lr=a.lr, threshold=a.threshold, fp16=a.fp16,
-                       log=sys.stderr)
+    rep, p, norm = run_study(a.train, a.test, a.epochs, lr=a.lr, threshold=a.threshold, fp16=a.fp16,
+                             log=sys.stderr)
     text = json.dumps(rep, indent=1)
     if a.out:
         with open(a.out, "w") as fh:
             fh.write(text + "\n")
     if a.params:
-        np.savez_compressed(a.params, params=p)
+        save_network(a.params, p, norm, K_DEFAULT, W_DEFAULT, a.threshold, a.train)
     print(text, flush=True)
     return 0
 

@@ -77,3 +77,21 @@ def test_failed_check_falls_back_to_the_full_run(trained) -> None:
     assert pred.fallback.all()
     assert np.array_equal(pred.s_params, rs.dft_full[idx])
     assert np.all(pred.steps == rs.n_full)
+
+
+def test_saved_network_predicts_the_same(trained, tmp_path) -> None:
+    """A network written by save_network and read back by load_network (parameters, descriptor
+    normaliser, constants; plain numpy arrays) gives the same predictions bit for bit."""
+    from open_analysis_amd.ringdown import extrapolate as X
+    from open_analysis_amd.ringdown import study
+
+    net, rs, norm, _, _ = trained
+    path = str(tmp_path / "net.npz")
+    study.save_network(path, net.params(), norm, study.K_DEFAULT, study.W_DEFAULT, 0.25, 96)
+    net2, norm2, meta = study.load_network(path, batch=net.S)
+    assert meta["K"] == study.K_DEFAULT and meta["train_geometries"] == 96
+    assert np.array_equal(net2.params(), net.params())
+    assert np.array_equal(norm2.mean, norm.mean) and np.array_equal(norm2.std, norm.std)
+    idx = np.arange(min(len(rs.driven), 64))
+    a, b = X.predict(net, rs, idx, norm), X.predict(net2, rs, idx, norm2)
+    assert np.array_equal(a.s_params, b.s_params) and np.array_equal(a.fallback, b.fallback)
