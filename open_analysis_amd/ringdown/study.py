@@ -86,7 +86,8 @@ def load_network(path: str, batch: int = 64) -> tuple[RingNet, X.Normaliser, dic
 def run_study(n_train: int, n_test: int, epochs: int, batch: int = 64, lr: float = 2e-3,
               K: int = K_DEFAULT, W: int = W_DEFAULT, n_full: int = N_DEFAULT,
               threshold: float = 0.25, fp16: bool = False, seed: int = 0, log: Any = None,
-              params: np.ndarray | None = None) -> tuple[dict[str, Any], np.ndarray, X.Normaliser]:
+              params: np.ndarray | None = None, save_to: str | None = None
+              ) -> tuple[dict[str, Any], np.ndarray, X.Normaliser]:
     t_all = time.perf_counter()
     m_tr, d_tr = geometry(n_train, 1000 + seed)
     m_te, d_te = geometry(n_test, 2000 + seed)
@@ -100,6 +101,8 @@ def run_study(n_train: int, n_test: int, epochs: int, batch: int = 64, lr: float
     hist = X.train(net, rs_tr, idx_tr, norm, epochs, lr=lr, seed=seed, log=log,
                    val=idx_te) if epochs > 0 else []
     t_train = time.perf_counter() - t0
+    if save_to:  # the trained network is written before the evaluation runs
+        save_network(save_to, net.params(), norm, K, W, threshold, n_train, seed)
     t0 = time.perf_counter()
     pred = X.predict(net, rs_te, idx_te, norm, threshold)
     t_pred = time.perf_counter() - t0
@@ -149,13 +152,11 @@ def main(argv: list[str] | None = None) -> int:
                     help="write the trained network, normaliser and constants (npz, see load_network)")
     a = ap.parse_args(argv)
     rep, p, norm = run_study(a.train, a.test, a.epochs, lr=a.lr, threshold=a.threshold, fp16=a.fp16,
-                             log=sys.stderr)
+                             log=sys.stderr, save_to=a.params)
     text = json.dumps(rep, indent=1)
     if a.out:
         with open(a.out, "w") as fh:
             fh.write(text + "\n")
-    if a.params:
-        save_network(a.params, p, norm, K_DEFAULT, W_DEFAULT, a.threshold, a.train)
     print(text, flush=True)
     return 0
 
