@@ -86,33 +86,50 @@ def load_network(path: str, batch: int = 64) -> tuple[RingNet, X.Normaliser, dic
 def run_study(n_train: int, n_test: int, epochs: int, batch: int = 64, lr: float = 2e-3,
               K: int = K_DEFAULT, W: int = W_DEFAULT, n_full: int = N_DEFAULT,
               threshold: float = 0.25, fp16: bool = False, seed: int = 0, log: Any = None,
-              params: np.ndarray | None = None, save_to: str | None = None
+              params: np.ndarray | None = None, save_to: str | None = None,
+              load: str | None = None, evaluate: bool = True
               ) -> tuple[dict[str, Any], np.ndarray, X.Normaliser]:
+    """Simulate, train and judge (see the module docstring). ``load`` takes a network written
+    by :func:`save_network` instead of simulating a training set and training; ``evaluate=False``
+    stops after training (and ``save_to``), so the two halves can run as separate jobs."""
     t_all = time.perf_counter()
-    m_tr, d_tr = geometry(n_train, 1000 + seed)
     m_te, d_te = geometry(n_test, 2000 + seed)
-    rs_tr = X.simulate(m_tr, d_tr, K, W, n_full, log=log)
-    rs_te = X.simulate(m_te, d_te, K, W, n_full, log=log)
-    norm = X.Normaliser.fit(rs_tr.geom)
-    net = RingNet(batch, X.time_len(K), FREQS.size, params=params, seed=seed)
-    idx_tr = np.arange(len(rs_tr.driven))
+    walls: dict[str, Any] = {}
+    if load:
+        net, norm, meta = load_network(load, batch)
+        K, W, n_train = meta["K"], meta["W"], meta["train_geometries"]
+        rs_te = X.simulate(m_te, d_te, K, W, n_full, log=log)
+        hist, t_train = [], 0.0
+    else:
+        m_tr, d_tr = geometry(n_train, 1000 + seed)
+        rs_tr = X.simulate(m_tr, d_tr, K, W, n_full, log=log)
+        rs_te = X.simulate(m_te, d_te, K, W, n_full, log=log)
+        walls["train"] = rs_tr.wall_s
+        norm = X.Normaliser.fit(rs_tr.geom)
+        net = RingNet(batch, X.time_len(K), FREQS.size, params=params, seed=seed)
+        idx_tr = np.arange(len(rs_tr.driven))
+        t0 = time.perf_counter()
+        hist = X.train(net, rs_tr, idx_tr, norm, epochs, lr=lr, seed=seed, log=log,
+                       val=np.arange(len(rs_te.driven))) if epochs > 0 else []
+        t_train = time.perf_counter() - t0
+        if save_to:  # the trained network is written before the evaluation runs
+            save_network(save_to, net.params(), norm, K, W, threshold, n_train, seed)
+    walls["test"] = rs_te.wall_s
     idx_te = np.arange(len(rs_te.driven))
-    t0 = time.perf_counter()
-    hist = X.train(net, rs_tr, idx_tr, norm, epochs, lr=lr, seed=seed, log=log,
-                   val=idx_te) if epochs > 0 else []
-    t_train = time.perf_counter() - t0
-    if save_to:  # the trained network is written before the evaluation runs
-        save_network(save_to, net.params(), norm, K, W, threshold, n_train, seed)
+    if not evaluate:
+        return ({"K": K, "W": W, "train_geometries": n_train, "epochs": epochs, "history": hist,
+                 "train_s": t_train, "simulate_s": walls, "total_s": time.perf_counter() - t_all},
+                net.params(), norm)
     t0 = time.perf_counter()
     pred = X.predict(net, rs_te, idx_te, norm, threshold)
     t_pred = time.perf_counter() - t0
     ref = rs_te.dft_full
     report: dict[str, Any] = {
         "K": K, "W": W, "n_full": n_full, "threshold": threshold,
-        "train_rows": int(len(idx_tr)), "test_rows": int(len(idx_te)),
+        "train_rows": int(4 * n_train), "test_rows": int(len(idx_te)),
         "train_geometries": n_train, "test_geometries": n_test, "epochs": epochs,
         "history": hist, "train_s": t_train, "predict_s": t_pred,
-        "simulate_s": {"train": rs_tr.wall_s, "test": rs_te.wall_s},
+        "simulate_s": walls, "loaded_network": load,
         "truncated": judge(rs_te, idx_te, rs_te.dft_kw, ref),
         "pencil": judge(rs_te, idx_te, X.pencil(rs_te, idx_te, start=pulse_end(m_te[0])), ref),
         "learned": judge(rs_te, idx_te, pred.s_params, ref),
@@ -148,11 +165,13 @@ def main(argv: list[str] | None = None) -> int:
     ap.add_argument("--threshold", type=float, default=0.25)
     ap.add_argument("--fp16", action="store_true")
     ap.add_argument("--out", default=None)
+    ap.add_argument("--load", default=None, help="judge a network written by --params (no training)")
+    ap.add_argument("--no-eval", action="store_true", help="stop after training and --params")
     ap.add_argument("--params", default=None,
                     help="write the trained network, normaliser and constants (npz, see load_network)")
     a = ap.parse_args(argv)
     rep, p, norm = run_study(a.train, a.test, a.epochs, lr=a.lr, threshold=a.threshold, fp16=a.fp16,
-                             log=sys.stderr, save_to=a.params)
+                             log=sys.stderr, save_to=a.params, load=a.load, evaluate=not a.no_eval)
     text = json.dumps(rep, indent=1)
     if a.out:
         with open(a.out, "w") as fh:
