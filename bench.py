@@ -12,6 +12,9 @@ divided by its mean launch time from HIP events on the engine's stream. ``cpu_ba
 float64 NumPy oracle stepping the same compiled problem on one host core. ``time_to_s21`` is the
 metric's second half: time to S21 within 0.05 dB, the learned ringdown extrapolator with FP16
 storage against the plain FP32 engine, on held-out via transitions (``--s21-geometries``).
+``--full-cpu-run STEPS`` adds the whole run on the HIP engine and on the float64 NumPy oracle,
+both timed, with their S-parameter difference (``--workload microstrip --full-cpu-run 8000`` is
+the 160x80x48 microstrip run to its full length).
 """
 
 from __future__ import annotations
@@ -90,6 +93,41 @@ def cpu_baseline(c, seconds: float) -> dict:
             "sample": f"float64 NumPy oracle, same compiled grid, {k} timed steps after 1 warm-up "
                       f"step ({wall:.1f} s), one core (numpy ufuncs are single-threaded; "
                       f"nproc={os.cpu_count()}, OMP_NUM_THREADS={os.environ.get('OMP_NUM_THREADS')})"}
+
+
+def full_run_parity(model, steps: int, dtype: str) -> dict:
+    """The whole run of ``model`` (``steps`` steps, no end criterion) on the HIP engine and on the
+    float64 NumPy oracle (one host core) in this process: both wall times, and the S-parameter
+    difference over every entry above -40 dB in dB and degrees (the 0.01 dB / 0.1 degree bar)."""
+    from open_analysis_amd.fdtd.run import run, sparams
+    from oracle import oracle_factory
+
+    last = [time.perf_counter()]
+
+    def progress(done: int) -> None:
+        if time.perf_counter() - last[0] > 30.0:
+            last[0] = time.perf_counter()
+            print(f"[bench] CPU oracle full run: {done}/{steps} steps", file=sys.stderr, flush=True)
+
+    t0 = time.perf_counter()
+    gpu = run(model, engine="hip", dtype=dtype, end_db=None, max_steps=steps)
+    t_gpu = time.perf_counter() - t0
+    print(f"[bench] HIP full run {t_gpu:.1f} s; CPU oracle full run starts", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    cpu = run(model, engine=oracle_factory, dtype="float64", end_db=None, max_steps=steps,
+              progress=progress)
+    t_cpu = time.perf_counter() - t0
+    s_gpu, s_cpu = sparams(model, gpu), sparams(model, cpu)
+    with np.errstate(divide="ignore"):
+        keep = 20 * np.log10(np.abs(s_cpu)) > -40.0  # entries of undriven columns are zero
+    ddb = np.abs(20 * np.log10(np.abs(s_gpu)) - 20 * np.log10(np.abs(s_cpu)))[keep]
+    dph = np.abs(np.angle(s_gpu / s_cpu, deg=True))[keep]
+    return {"steps": steps, "dtype": dtype, "excitations": int(model.drive_matrix().shape[0]),
+            "hip_s": t_gpu, "cpu_oracle_s": t_cpu, "cpu_cores": 1, "speedup": t_cpu / t_gpu,
+            "note": "whole run() each, compile_model and the S-parameter DFTs included",
+            "s_err_db_max": float(ddb.max()), "s_err_deg_max": float(dph.max()),
+            "entries_compared": int(keep.sum()),
+            "within_0.01_db_0.1_deg": bool(ddb.max() < 0.01 and dph.max() < 0.1)}
 
 
 def copy_rate(nbytes: int, reps: int) -> dict:
@@ -223,6 +261,9 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--geometries", type=int, default=32, help="via_batch: number of geometries")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--full-cpu-run", type=int, default=0, metavar="STEPS",
+                    help="also run the whole workload for STEPS steps on the HIP engine and on the "
+                         "float64 NumPy oracle, reporting both times and the S difference")
     ap.add_argument("--s21-geometries", type=int, default=16,
                     help="held-out via geometries of the time-to-S21 leg (0 skips it)")
     ap.add_argument("--calibrate-copy", type=float, default=0.0, metavar="GIB",
@@ -295,6 +336,11 @@ def main() -> int:
     if not a.no_cpu:
         print("[bench] timing the CPU oracle", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(c, a.cpu_seconds)
+    if a.full_cpu_run > 0:
+        pm, _ = build_workload(a.workload)
+        if pm.ports:  # port 1 driven: S11 and S21, the quantities the reference reports
+            pm.excite = [pm.ports[0].name]
+        out["full_run_vs_cpu_oracle"] = full_run_parity(pm, a.full_cpu_run, a.dtype)
     print(json.dumps(out), flush=True)
     return 0
 
