@@ -118,10 +118,10 @@ def full_run_parity(model, steps: int, dtype: str) -> dict:
               progress=progress)
     t_cpu = time.perf_counter() - t0
     s_gpu, s_cpu = sparams(model, gpu), sparams(model, cpu)
-    with np.errstate(divide="ignore"):
-        keep = 20 * np.log10(np.abs(s_cpu)) > -40.0  # entries of undriven columns are zero
-    ddb = np.abs(20 * np.log10(np.abs(s_gpu)) - 20 * np.log10(np.abs(s_cpu)))[keep]
-    dph = np.abs(np.angle(s_gpu / s_cpu, deg=True))[keep]
+    with np.errstate(divide="ignore", invalid="ignore"):  # undriven columns are zero
+        keep = 20 * np.log10(np.abs(s_cpu)) > -40.0
+        ddb = np.abs(20 * np.log10(np.abs(s_gpu)) - 20 * np.log10(np.abs(s_cpu)))[keep]
+        dph = np.abs(np.angle(s_gpu / s_cpu, deg=True))[keep]
     return {"steps": steps, "dtype": dtype, "excitations": int(model.drive_matrix().shape[0]),
             "hip_s": t_gpu, "cpu_oracle_s": t_cpu, "cpu_cores": 1, "speedup": t_cpu / t_gpu,
             "note": "whole run() each, compile_model and the S-parameter DFTs included",
