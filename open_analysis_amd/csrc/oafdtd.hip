@@ -323,6 +323,7 @@ struct Engine final : oafd_engine {
         }
         A.ca = dev_vec(cast_vec<R>(d->ca, d->n_mat));
         A.cb = dev_vec(cast_vec<R>(d->cb, d->n_mat));
+        A.ntab = d->n_mat;
         const int len[3] = {X, Y, Z};
         for (int a = 0; a < 3; ++a) {
             A.ip[a] = dev_vec(cast_vec<R>(d->inv_primal[a], len[a]));

@@ -37,6 +37,15 @@ namespace oafd {
 constexpr int TXH = 64;       // H columns per tile = one wavefront
 constexpr int TXE = TXH - 1;  // E columns per tile
 constexpr int NXCD = 8;       // XCDs of an MI355X: workgroup L runs on XCD L % 8
+constexpr int NTAB = 256;     // material rows held in LDS (larger tables are read from memory)
+
+// Read-only value at a workgroup-uniform index, loaded through the constant address space so
+// that it is a scalar load. Vector loads complete in order: a uniform value fetched with a
+// vector load after the next plane's prefetch makes its first use wait for that whole prefetch.
+template <typename T>
+__device__ __forceinline__ T ldu(const T* p, long long i) {
+    return ((const __attribute__((address_space(4))) T*)p)[i];
+}
 
 template <typename R>
 struct YeeArgs {
@@ -51,6 +60,7 @@ struct YeeArgs {
     const uint16_t* idx[3];
     const R* ca;
     const R* cb;
+    int ntab;  // rows of ca, cb
     const R* ip[3];
     const R* id[3];
     const int* kb;       // z chunk c covers storage planes [kb[c], kb[c+1])
@@ -95,9 +105,18 @@ __device__ __forceinline__ Slab<R> slab_at(const YeeArgs<R>& A, int a, bool hkin
     Slab<R> o{-1, (R)0, (R)0, (R)0};
     const int len = a == 0 ? A.X : (a == 1 ? A.Y : A.Z);
     if (A.map[a] == nullptr || I < 0 || I >= len) return o;
+    const R* cf = A.coef[a] + (hkind ? 3 * len : 0);
+    if (a == 2) {  // z: one plane per workgroup, scalar loads
+        o.s = ldu(A.map[a], (hkind ? len : 0) + I);
+        if (o.s >= 0) {
+            o.b = ldu(cf, I);
+            o.c = ldu(cf, len + I);
+            o.k = ldu(cf, 2 * len + I);
+        }
+        return o;
+    }
     o.s = A.map[a][(hkind ? len : 0) + I];
     if (o.s >= 0) {
-        const R* cf = A.coef[a] + (hkind ? 3 * len : 0);
         o.b = cf[I];
         o.c = cf[len + I];
         o.k = cf[2 * len + I];
@@ -114,10 +133,13 @@ __device__ __forceinline__ void cpml_term(R& d, const Slab<R>& sl, const R* pin,
 }
 
 // One tile: E cells [1 + 63 tx, +63) x [1 + (TYH-1) ty, +TYH-1) of member b, z chunk zc.
-// sE / sH are the workgroup's LDS slots (E^n(k) with its high halo; H^{n+1/2}(k)).
-template <typename S, typename R, int NW, int RPW, bool GEN, bool ZS>
+// sE / sH are the workgroup's LDS slots (E^n(k) with its high halo; H^{n+1/2}(k)). TAB: the
+// material rows (ca, cb and the single-pole beta) are copied to sT and read from LDS, so that
+// no vector load in the z loop is issued after the prefetch it would have to wait behind.
+template <typename S, typename R, int NW, int RPW, bool GEN, bool ZS, bool TAB>
 __device__ __forceinline__ void yee_tile(const YeeArgs<R>& A, int tx, int ty, int zc, int b,
-                                         R (&sE)[3][NW * RPW + 1][TXH + 1], R (&sH)[3][NW * RPW][TXH]) {
+                                         R (&sE)[3][NW * RPW + 1][TXH + 1], R (&sH)[3][NW * RPW][TXH],
+                                         R (&sT)[3][NTAB]) {
     constexpr int TYH = NW * RPW;  // H rows per tile
     constexpr int TYE = TYH - 1;   // E rows per tile
     const int lane = threadIdx.x, wv = threadIdx.y;
@@ -182,7 +204,25 @@ __device__ __forceinline__ void yee_tile(const YeeArgs<R>& A, int tx, int ty, in
     };
     R e_nx[RPW][3], e_own[RPW][3], row_nx[3], col_nx = 0;
     S e_pf[RPW][3], row_pf[3], col_pf = zs;  // E(k+2) in flight, in storage type
+    // Interior and z-slab tiles issue every prefetch load unconditionally (planes past the
+    // storage are read at Z-1 and zeroed in widen): the compiler's wait counts treat a load
+    // under a branch as not issued, so one conditional prefetch would make the next use of any
+    // older load wait for the whole prefetch.
     auto load_plane = [&](int kk, S (&e)[RPW][3], S (&row)[3], S& col) {
+        if constexpr (!GEN) {
+            const int kc = kk < Z ? kk : Z - 1;
+#pragma unroll
+            for (int r = 0; r < RPW; ++r)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) e[r][c] = ldraw<S>(ein[c], kc * sxy + jr[r] * X + i);
+            // the last wave reads the halo row, the others their first row again (not staged)
+            const int jrow = row_halo ? jh : jr[0];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) row[c] = ldraw<S>(ein[c], kc * sxy + jrow * X + i);
+            // threads past the halo column read their own Ex (not staged)
+            col = ldraw<S>(ein[col_halo ? hc_c : 0], kc * sxy + (col_halo ? jhc : jr[0]) * X + (col_halo ? ih : i));
+            return;
+        }
         const bool kin = kk < Z;
 #pragma unroll
         for (int r = 0; r < RPW; ++r)
@@ -194,14 +234,15 @@ __device__ __forceinline__ void yee_tile(const YeeArgs<R>& A, int tx, int ty, in
         }
         if (col_halo) col = kin ? ld_e(hc_c, ih, jhc, kk) : zs;
     };
-    auto widen = [&](const S (&e)[RPW][3], const S (&row)[3], S col, R (&eo)[RPW][3], R (&ro)[3], R& co) {
+    auto widen = [&](const S (&e)[RPW][3], const S (&row)[3], S col, R (&eo)[RPW][3], R (&ro)[3], R& co,
+                     bool in) {  // in: the plane lies in the storage (else E is zero)
 #pragma unroll
         for (int r = 0; r < RPW; ++r)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) eo[r][c] = cvt<S, R>(e[r][c]);
+            for (int c = 0; c < 3; ++c) eo[r][c] = in ? cvt<S, R>(e[r][c]) : (R)0;
 #pragma unroll
-        for (int c = 0; c < 3; ++c) ro[c] = cvt<S, R>(row[c]);
-        co = cvt<S, R>(col);
+        for (int c = 0; c < 3; ++c) ro[c] = in ? cvt<S, R>(row[c]) : (R)0;
+        co = in ? cvt<S, R>(col) : (R)0;
     };
     auto stage = [&](const R (&e)[RPW][3], const R (&row)[3], R col) {
 #pragma unroll
@@ -216,12 +257,39 @@ __device__ __forceinline__ void yee_tile(const YeeArgs<R>& A, int tx, int ty, in
     };
     // H^{n-1/2}, material rows and the single-pole Debye current of one plane
     S hold_pf[RPW][3];
-    R jj_pf[RPW][3];
+    R jj_pf[RPW][3] = {};
     int mat_pf[RPW][3];
     const bool ade1 = A.npole == 1;
     const long long jstride = (long long)A.njk * sxy;
     const R* Jb = ade1 ? A.J + (long long)b * 3 * jstride : nullptr;  // pole 0 of member b
+    const R kp0 = ade1 ? ldu(A.kp, 0) : (R)0, kq0 = ade1 ? ldu(A.kq, 0) : (R)0;
+    if constexpr (TAB) {  // read first in the E update, two barriers into the z loop
+        for (int t = tid; t < A.ntab; t += TXH * NW) {
+            sT[0][t] = A.ca[t];
+            sT[1][t] = A.cb[t];
+            sT[2][t] = (ade1 && t >= A.ade_first) ? A.beta[t - A.ade_first] : (R)0;
+        }
+    }
     auto load_cells = [&](int kk) {
+        if constexpr (!GEN) {
+            // H and the rows of plane kk >= kend are never used; rows and Debye current are
+            // read for every lane and used by the E-updating ones, the current only in Debye
+            // cells, whose planes all lie in [jk0, jk0 + njk)
+            const int kc = kk < Z ? kk : Z - 1;
+            const int kj = kk < A.jk0 ? A.jk0 : (kk >= A.jk0 + A.njk ? A.jk0 + A.njk - 1 : kk);
+#pragma unroll
+            for (int r = 0; r < RPW; ++r) {
+                const int q = kc * sxy + jr[r] * X + i;
+                const int jq = (kj - A.jk0) * sxy + jr[r] * X + i;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    hold_pf[r][c] = ldraw<S>(hin[c], q);
+                    mat_pf[r][c] = (int)A.idx[c][q];
+                    if (ade1) jj_pf[r][c] = Jb[c * (int)jstride + jq];
+                }
+            }
+            return;
+        }
         const bool kin = kk < kend;
         const bool kown = kk >= kbeg;  // the halo plane kbeg-1 only needs H
         const bool jin = ade1 && kin && kk >= A.jk0 && kk < A.jk0 + A.njk;
@@ -245,10 +313,10 @@ __device__ __forceinline__ void yee_tile(const YeeArgs<R>& A, int tx, int ty, in
         S e3[RPW][3], r3[3], c3 = zs;
         R r3w[3], clw = 0;
         load_plane(kstart, e3, r3, c3);
-        widen(e3, r3, c3, e_own, r3w, clw);
+        widen(e3, r3, c3, e_own, r3w, clw, kstart < Z);
         stage(e_own, r3w, clw);
         load_plane(kstart + 1, e3, r3, c3);
-        widen(e3, r3, c3, e_nx, row_nx, col_nx);
+        widen(e3, r3, c3, e_nx, row_nx, col_nx, kstart + 1 < Z);
     }
     load_cells(kstart);
 
@@ -270,9 +338,9 @@ __device__ __forceinline__ void yee_tile(const YeeArgs<R>& A, int tx, int ty, in
                 mat[r][c] = mat_pf[r][c];
                 jcur[r][c] = jj_pf[r][c];
             }
-        if (k + 2 <= kend) load_plane(k + 2, e_pf, row_pf, col_pf);  // H(kend-1) needs E(kend)
+        if (!GEN || k + 2 <= kend) load_plane(k + 2, e_pf, row_pf, col_pf);  // H(kend-1) needs E(kend)
         load_cells(k + 1);
-        const R ipz = A.ip[2][k], idz = A.id[2][k];
+        const R ipz = ldu(A.ip[2], k), idz = ldu(A.id[2], k);
         Slab<R> szE, szH;
         if constexpr (GEN || ZS) {
             szE = slab_at(A, 2, false, (!GEN || slab_z) ? k : -1);
@@ -473,20 +541,22 @@ __device__ __forceinline__ void yee_tile(const YeeArgs<R>& A, int tx, int ty, in
                 for (int c = 0; c < 3; ++c) {
                     const int m = mat[r][c];
                     const R eold = e_own[r][c];
-                    R en = A.ca[m] * eold + A.cb[m] * ((d[c][0] - d[c][1]) * A.inv_hs);
+                    const R cam = TAB ? sT[0][m] : A.ca[m], cbm = TAB ? sT[1][m] : A.cb[m];
+                    R en = cam * eold + cbm * ((d[c][0] - d[c][1]) * A.inv_hs);
                     if (m >= A.ade_first) {
                         // E -= cb sum_p (1+k_p)/2 J_p ;  J_p = k_p J_p + beta_p (E^{n+1} - E^n)
                         const int row = m - A.ade_first;
                         const int jq = (k - A.jk0) * sxy + j * X + i;
                         if (ade1) {
                             const R jv = jcur[r][c];
-                            en = en - A.cb[m] * (A.kq[0] * jv);
-                            const_cast<R*>(Jb)[c * (int)jstride + jq] = A.kp[0] * jv + A.beta[row] * (en - eold);
+                            en = en - cbm * (kq0 * jv);
+                            const R bt = TAB ? sT[2][m] : A.beta[row];
+                            const_cast<R*>(Jb)[c * (int)jstride + jq] = kp0 * jv + bt * (en - eold);
                         } else {
                             R jt = (R)0;
                             for (int p = 0; p < A.npole; ++p)
                                 jt += A.kq[p] * A.J[((long long)(p * A.B + b) * 3 + c) * jstride + jq];
-                            en = en - A.cb[m] * jt;
+                            en = en - cbm * jt;
                             const R de = en - eold;
                             for (int p = 0; p < A.npole; ++p) {
                                 R* jp = A.J + ((long long)(p * A.B + b) * 3 + c) * jstride + jq;
@@ -505,7 +575,7 @@ __device__ __forceinline__ void yee_tile(const YeeArgs<R>& A, int tx, int ty, in
                 hprev[r][c] = hcur[r][c];
                 e_own[r][c] = e_nx[r][c];
             }
-        widen(e_pf, row_pf, col_pf, e_nx, row_nx, col_nx);
+        widen(e_pf, row_pf, col_pf, e_nx, row_nx, col_nx, k + 2 < Z);
     }
 }
 
@@ -515,6 +585,7 @@ __global__ __launch_bounds__(TXH * NW) void yee_fused(YeeArgs<R> A) {
     // slab applied per plane, no wall ghosts (z walls are CPML or PEC).
     __shared__ R sE[3][NW * RPW + 1][TXH + 1];
     __shared__ R sH[3][NW * RPW][TXH];
+    __shared__ R sT[3][NTAB];
     // ---- XCD-aware tile: XCD x runs the contiguous tile range [x*per, (x+1)*per), so tiles
     // that are neighbours in the launch order (in x or in y, as planned) share halo lines in L2
     const int per = (A.ntiles + NXCD - 1) / NXCD;
@@ -536,7 +607,10 @@ __global__ __launch_bounds__(TXH * NW) void yee_fused(YeeArgs<R> A) {
         zc = A.zcl[r % A.nzc];
         b = r / A.nzc;
     }
-    yee_tile<S, R, NW, RPW, GEN, ZS>(A, tx, ty, zc, b, sE, sH);
+    if (A.ntab <= NTAB)
+        yee_tile<S, R, NW, RPW, GEN, ZS, true>(A, tx, ty, zc, b, sE, sH, sT);
+    else
+        yee_tile<S, R, NW, RPW, GEN, ZS, false>(A, tx, ty, zc, b, sE, sH, sT);
 }
 
 // Persistent step over the tiles of many problems (uneven grids) in one launch. list holds
@@ -551,6 +625,7 @@ __global__ __launch_bounds__(TXH * NW) void yee_batch(const YeeArgs<R>* __restri
                                                       unsigned* ctr, int parity) {
     __shared__ R sE[3][NW * RPW + 1][TXH + 1];
     __shared__ R sH[3][NW * RPW][TXH];
+    __shared__ R sT[3][NTAB];
     __shared__ int ticket;
     const int x = (int)(blockIdx.x % NXCD);
     const bool lead = threadIdx.x == 0 && threadIdx.y == 0;
@@ -566,12 +641,19 @@ __global__ __launch_bounds__(TXH * NW) void yee_batch(const YeeArgs<R>* __restri
         const int4 e = list[beg + t];
         const int kind = e.x >> 16, tx = e.x & 0xffff, ty = e.y, zc = e.z, b = e.w & 0xff;
         const YeeArgs<R>& A = args[e.w >> 8];
-        if (kind == 0)
-            yee_tile<S, R, NW, RPW, false, false>(A, tx, ty, zc, b, sE, sH);
+        if (A.ntab > NTAB) {
+            if (kind == 0)
+                yee_tile<S, R, NW, RPW, false, false, false>(A, tx, ty, zc, b, sE, sH, sT);
+            else if (kind == 1)
+                yee_tile<S, R, NW, RPW, false, true, false>(A, tx, ty, zc, b, sE, sH, sT);
+            else
+                yee_tile<S, R, NW, RPW, true, false, false>(A, tx, ty, zc, b, sE, sH, sT);
+        } else if (kind == 0)
+            yee_tile<S, R, NW, RPW, false, false, true>(A, tx, ty, zc, b, sE, sH, sT);
         else if (kind == 1)
-            yee_tile<S, R, NW, RPW, false, true>(A, tx, ty, zc, b, sE, sH);
+            yee_tile<S, R, NW, RPW, false, true, true>(A, tx, ty, zc, b, sE, sH, sT);
         else
-            yee_tile<S, R, NW, RPW, true, false>(A, tx, ty, zc, b, sE, sH);
+            yee_tile<S, R, NW, RPW, true, false, true>(A, tx, ty, zc, b, sE, sH, sT);
     }
 }
 
