@@ -204,13 +204,15 @@ __device__ __forceinline__ void yee_tile(const YeeArgs<R>& A, int tx, int ty, in
     };
     R e_nx[RPW][3], e_own[RPW][3], row_nx[3], col_nx = 0;
     S e_pf[RPW][3], row_pf[3], col_pf = zs;  // E(k+2) in flight, in storage type
-    // Interior and z-slab tiles issue every prefetch load unconditionally (planes past the
-    // storage are read at Z-1 and zeroed in widen): the compiler's wait counts treat a load
-    // under a branch as not issued, so one conditional prefetch would make the next use of any
-    // older load wait for the whole prefetch.
+    // Interior and z-slab tiles issue every prefetch load unconditionally: the compiler's wait
+    // counts treat a load under a branch as not issued, so one conditional prefetch would make
+    // the next use of any older load wait for the whole prefetch. Planes past the chunk (never
+    // used) re-read its last plane, which is still in L2; planes past the storage are zeroed in
+    // widen.
     auto load_plane = [&](int kk, S (&e)[RPW][3], S (&row)[3], S& col) {
         if constexpr (!GEN) {
-            const int kc = kk < Z ? kk : Z - 1;
+            int kc = kk < kend ? kk : kend;
+            if (kc > Z - 1) kc = Z - 1;
 #pragma unroll
             for (int r = 0; r < RPW; ++r)
 #pragma unroll
@@ -272,10 +274,11 @@ __device__ __forceinline__ void yee_tile(const YeeArgs<R>& A, int tx, int ty, in
     }
     auto load_cells = [&](int kk) {
         if constexpr (!GEN) {
-            // H and the rows of plane kk >= kend are never used; rows and Debye current are
-            // read for every lane and used by the E-updating ones, the current only in Debye
-            // cells, whose planes all lie in [jk0, jk0 + njk)
-            const int kc = kk < Z ? kk : Z - 1;
+            // H and the rows of plane kk >= kend are never used (plane kend-1 is re-read from
+            // L2 instead); rows and Debye current are read for every lane and used by the
+            // E-updating ones, the current only in Debye cells, whose planes all lie in
+            // [jk0, jk0 + njk)
+            const int kc = kk < kend ? kk : kend - 1;
             const int kj = kk < A.jk0 ? A.jk0 : (kk >= A.jk0 + A.njk ? A.jk0 + A.njk - 1 : kk);
 #pragma unroll
             for (int r = 0; r < RPW; ++r) {
