@@ -135,6 +135,24 @@ def interior_block(k: Classes) -> Any:
     return m
 
 
+def many_materials(k: Classes) -> Any:
+    """interior_block with 90 dielectric blocks of distinct permittivity (10 x 10 x 13 cells,
+    eps 1.5 .. 4.8) under its Debye slab: the averaged edges at the block faces give a material
+    table of several hundred rows."""
+    d = 0.5e-3
+    m = interior_block(k)
+    blocks = []
+    for ix in range(15):
+        for iy in range(3):
+            for iz in range(2):
+                n = (ix * 3 + iy) * 2 + iz
+                lo = (ix * 10 * d, iy * 10 * d, iz * 13 * d)
+                hi = ((ix + 1) * 10 * d, (iy + 1) * 10 * d, (iz + 1) * 13 * d)
+                blocks.append(k["Region"](lo, hi, 1.5 + 0.037 * n))
+    m.regions[0:0] = blocks
+    return m
+
+
 def tile_seam(k: Classes, nx: int, walls: tuple[str, ...]) -> Any:
     """nx x 20 x 14 uniform 0.5 mm mesh with a 4-cell CPML where the walls ask for one; the
     caller picks nx so that an x tile seam falls on boundary code. A Debye slab, two soft sources

@@ -132,6 +132,36 @@ def test_tile_seams_match_oracle(name: str, dtype: str, monkeypatch) -> None:
         assert np.array_equal(got[0][k], got[1][k]), k
 
 
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+def test_large_material_table_matches_oracle(dtype: str, monkeypatch) -> None:
+    """A material table too large for the kernels' LDS copy (428 rows, 256 fit) is read from
+    memory by a second instantiation of the tile code: fields after 120 steps against the
+    float64 oracle, for two tile shapes, which give the same bits."""
+    import cases
+    from golden_io import OWN_CLASSES
+
+    from open_analysis_amd.fdtd.engine import extended_tables
+
+    c = compile_model(cases.many_materials(OWN_CLASSES))
+    t = extended_tables(c)
+    assert t["n_mat"] > 256 and t["n_poles"] == 1 and t["ade_first"] < t["n_mat"]
+    ref = OracleEngine(c, "float64")
+    ref.step(120)
+    want = ref.get_fields()
+    got = []
+    for variant in (0, 6):
+        _say(f"[large-table] dtype={dtype} variant={variant}")
+        monkeypatch.setenv("OAFD_YEE_VARIANT", str(variant))
+        eng = HipEngine(c, dtype, max_steps=200)
+        eng.step(120)
+        f = eng.get_fields()
+        err = _fields_err(f, want)
+        assert max(err.values()) < TOL[dtype], (variant, err)
+        got.append(f)
+    for k in got[0]:
+        assert np.array_equal(got[0][k], got[1][k]), k
+
+
 def test_graph_equals_eager() -> None:
     """Same device, same input: the 64-step graph and eager launches are bitwise identical
     (test_gpu_cpu.py:86-96)."""
