@@ -71,6 +71,29 @@ def test_batch_members_match_single_runs_in_every_storage_form(small_set, dtype:
         _assert_same(_state(batch.members[i]), _state(eng))
 
 
+@pytest.mark.parametrize("force", ["", "6,1,8"])
+def test_batch_with_interior_tiles_matches_single_runs(small_set, force: str, monkeypatch) -> None:
+    """Members large enough for interior and z-slab tiles (the small via grids have only general
+    tiles), one with a 428-row material table (read from memory instead of LDS), beside a via
+    geometry: each equals its own engine bit for bit, with the default plan and with 16x1 tiles
+    in 8 interior z chunks."""
+    import cases
+    from golden_io import OWN_CLASSES
+
+    if force:
+        monkeypatch.setenv("OAFD_YEE_VARIANT", force)
+    _, vcs = small_set
+    cs = [compile_model(cases.interior_block(OWN_CLASSES)),
+          compile_model(cases.many_materials(OWN_CLASSES)), vcs[0]]
+    batch = HipBatch(cs, "float32", max_steps=160)
+    batch.step(160)
+    for i, c in enumerate(cs):
+        _say(f"[batch] interior-tile member {i} vs its own engine, plan '{force}'")
+        eng = HipEngine(c, "float32", max_steps=160)
+        eng.step(160)
+        _assert_same(_state(batch.members[i]), _state(eng))
+
+
 def test_subset_batch_is_invariant(small_set) -> None:
     _, cs = small_set
     full = HipBatch(cs, "float32", max_steps=STEPS)
