@@ -23,11 +23,19 @@ __device__ __forceinline__ void stS(void* base, I i, R v) {
         static_cast<S*>(base)[i] = (S)v;
 }
 
+// Load from device memory through the global address space. A pointer picked at run time from
+// a small array loses its address space, and the compiler then emits a FLAT load, which also
+// counts on the LDS/scalar counter: the next wait for an LDS read would wait for it as well.
+template <typename T, typename I = long long>
+__device__ __forceinline__ T ldg(const T* base, I i) {
+    return ((const __attribute__((address_space(1))) T*)base)[i];
+}
+
 // Raw storage load (no conversion) and the conversion on its own: prefetched 16-bit values stay
 // packed in registers until they are used, so the load's wait is not forced at issue.
 template <typename S, typename I = long long>
 __device__ __forceinline__ S ldraw(const void* base, I i) {
-    return static_cast<const S*>(base)[i];
+    return ldg(static_cast<const S*>(base), i);
 }
 template <typename S, typename R>
 __device__ __forceinline__ R cvt(S v) {

@@ -124,10 +124,11 @@ __device__ __forceinline__ Slab<R> slab_at(const YeeArgs<R>& A, int a, bool hkin
     return o;
 }
 
-// psi <- b psi + c d ;  d <- (1 + k) d + psi   (Roden-Gedney CPML, pml.py:90-100)
+// psi <- b psi + c d ;  d <- (1 + k) d + psi   (Roden-Gedney CPML, pml.py:90-100); pin is the
+// psi value already loaded, the new one is stored at pout[q] when write
 template <typename R>
-__device__ __forceinline__ void cpml_term(R& d, const Slab<R>& sl, const R* pin, R* pout, long long q, bool write) {
-    const R psi = sl.b * pin[q] + sl.c * d;
+__device__ __forceinline__ void cpml_term(R& d, const Slab<R>& sl, R pin, R* pout, long long q, bool write) {
+    const R psi = sl.b * pin + sl.c * d;
     if (write) pout[q] = psi;
     d = d + sl.k * d + psi;
 }
@@ -198,53 +199,42 @@ __device__ __forceinline__ void yee_tile(const YeeArgs<R>& A, int tx, int ty, in
     const int ih = i0 - 1 + TXH, jhc = j0 - 1 + hc_lj;
 
     const S zs = static_cast<S>(0.0f);
-    auto ld_e = [&](int c, int ii, int jj, int kk) -> S {  // raw storage value, 0 outside
-        if (GEN && (ii >= X || jj >= Y || kk >= Z)) return zs;
-        return ldraw<S>(ein[c], kk * sxy + jj * X + ii);
-    };
     R e_nx[RPW][3], e_own[RPW][3], row_nx[3], col_nx = 0;
     S e_pf[RPW][3], row_pf[3], col_pf = zs;  // E(k+2) in flight, in storage type
-    // Interior and z-slab tiles issue every prefetch load unconditionally: the compiler's wait
-    // counts treat a load under a branch as not issued, so one conditional prefetch would make
-    // the next use of any older load wait for the whole prefetch. Planes past the chunk (never
-    // used) re-read its last plane, which is still in L2; planes past the storage are zeroed in
-    // widen.
+    // Every prefetch load is issued unconditionally, at an address clamped into the storage: the
+    // compiler's wait counts treat a load under a branch as not issued, so one conditional
+    // prefetch would make the next use of any older load wait for the whole prefetch. Planes past
+    // the chunk (never used) re-read its last plane, which is still in L2; values outside the
+    // storage are zeroed in widen.
+    const int ic = i < X ? i : X - 1;
     auto load_plane = [&](int kk, S (&e)[RPW][3], S (&row)[3], S& col) {
-        if constexpr (!GEN) {
-            int kc = kk < kend ? kk : kend;
-            if (kc > Z - 1) kc = Z - 1;
+        int kc = kk < kend ? kk : kend;
+        if (kc > Z - 1) kc = Z - 1;
 #pragma unroll
-            for (int r = 0; r < RPW; ++r)
+        for (int r = 0; r < RPW; ++r) {
+            const int jc = jr[r] < Y ? jr[r] : Y - 1;
 #pragma unroll
-                for (int c = 0; c < 3; ++c) e[r][c] = ldraw<S>(ein[c], kc * sxy + jr[r] * X + i);
-            // the last wave reads the halo row, the others their first row again (not staged)
-            const int jrow = row_halo ? jh : jr[0];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) row[c] = ldraw<S>(ein[c], kc * sxy + jrow * X + i);
-            // threads past the halo column read their own Ex (not staged)
-            col = ldraw<S>(ein[col_halo ? hc_c : 0], kc * sxy + (col_halo ? jhc : jr[0]) * X + (col_halo ? ih : i));
-            return;
+            for (int c = 0; c < 3; ++c) e[r][c] = ldraw<S>(ein[c], kc * sxy + jc * X + ic);
         }
+        // the last wave reads the halo row, the others their first row again (not staged)
+        const int jrow = row_halo ? (jh < Y ? jh : Y - 1) : (jr[0] < Y ? jr[0] : Y - 1);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) row[c] = ldraw<S>(ein[c], kc * sxy + jrow * X + ic);
+        // threads past the halo column read their own Ex (not staged)
+        const int cj = col_halo ? jhc : jr[0], ci = col_halo ? ih : i;
+        col = ldraw<S>(ein[col_halo ? hc_c : 0], kc * sxy + (cj < Y ? cj : Y - 1) * X + (ci < X ? ci : X - 1));
+    };
+    // widen plane kk: E outside the storage is zero
+    auto widen = [&](const S (&e)[RPW][3], const S (&row)[3], S col, R (&eo)[RPW][3], R (&ro)[3], R& co,
+                     int kk) {
         const bool kin = kk < Z;
 #pragma unroll
         for (int r = 0; r < RPW; ++r)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) e[r][c] = kin ? ld_e(c, i, jr[r], kk) : zs;
-        if (row_halo) {
+            for (int c = 0; c < 3; ++c) eo[r][c] = (kin && i < X && jr[r] < Y) ? cvt<S, R>(e[r][c]) : (R)0;
 #pragma unroll
-            for (int c = 0; c < 3; ++c) row[c] = kin ? ld_e(c, i, jh, kk) : zs;
-        }
-        if (col_halo) col = kin ? ld_e(hc_c, ih, jhc, kk) : zs;
-    };
-    auto widen = [&](const S (&e)[RPW][3], const S (&row)[3], S col, R (&eo)[RPW][3], R (&ro)[3], R& co,
-                     bool in) {  // in: the plane lies in the storage (else E is zero)
-#pragma unroll
-        for (int r = 0; r < RPW; ++r)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) eo[r][c] = in ? cvt<S, R>(e[r][c]) : (R)0;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) ro[c] = in ? cvt<S, R>(row[c]) : (R)0;
-        co = in ? cvt<S, R>(col) : (R)0;
+        for (int c = 0; c < 3; ++c) ro[c] = (kin && i < X && jh < Y) ? cvt<S, R>(row[c]) : (R)0;
+        co = (kin && ih < X && jhc < Y) ? cvt<S, R>(col) : (R)0;
     };
     auto stage = [&](const R (&e)[RPW][3], const R (&row)[3], R col) {
 #pragma unroll
@@ -272,41 +262,23 @@ __device__ __forceinline__ void yee_tile(const YeeArgs<R>& A, int tx, int ty, in
             sT[2][t] = (ade1 && t >= A.ade_first) ? A.beta[t - A.ade_first] : (R)0;
         }
     }
+    // H and the rows of plane kk >= kend are never used (plane kend-1 is re-read from L2
+    // instead). H of a lane outside the storage is never used (its H is set to zero, invalid),
+    // rows and Debye current only by the lanes that update E, the current only in Debye cells,
+    // whose planes all lie in [jk0, jk0 + njk).
     auto load_cells = [&](int kk) {
-        if constexpr (!GEN) {
-            // H and the rows of plane kk >= kend are never used (plane kend-1 is re-read from
-            // L2 instead); rows and Debye current are read for every lane and used by the
-            // E-updating ones, the current only in Debye cells, whose planes all lie in
-            // [jk0, jk0 + njk)
-            const int kc = kk < kend ? kk : kend - 1;
-            const int kj = kk < A.jk0 ? A.jk0 : (kk >= A.jk0 + A.njk ? A.jk0 + A.njk - 1 : kk);
-#pragma unroll
-            for (int r = 0; r < RPW; ++r) {
-                const int q = kc * sxy + jr[r] * X + i;
-                const int jq = (kj - A.jk0) * sxy + jr[r] * X + i;
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    hold_pf[r][c] = ldraw<S>(hin[c], q);
-                    mat_pf[r][c] = (int)A.idx[c][q];
-                    if (ade1) jj_pf[r][c] = Jb[c * (int)jstride + jq];
-                }
-            }
-            return;
-        }
-        const bool kin = kk < kend;
-        const bool kown = kk >= kbeg;  // the halo plane kbeg-1 only needs H
-        const bool jin = ade1 && kin && kk >= A.jk0 && kk < A.jk0 + A.njk;
+        const int kc = kk < kend ? kk : kend - 1;
+        const int kj = kk < A.jk0 ? A.jk0 : (kk >= A.jk0 + A.njk ? A.jk0 + A.njk - 1 : kk);
 #pragma unroll
         for (int r = 0; r < RPW; ++r) {
-            const bool ok = kin && col_ok && (!GEN || jr[r] < Y);
-            const bool ecell = ok && kown && lane >= 1 && wv * RPW + r >= 1;  // index, J feed E only
-            const int q = kk * sxy + jr[r] * X + i;
-            const int jq = (kk - A.jk0) * sxy + jr[r] * X + i;
+            const int jc = jr[r] < Y ? jr[r] : Y - 1;
+            const int q = kc * sxy + jc * X + ic;
+            const int jq = (kj - A.jk0) * sxy + jc * X + ic;
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                hold_pf[r][c] = ok ? ldraw<S>(hin[c], q) : zs;
-                mat_pf[r][c] = ecell ? (int)A.idx[c][q] : 0;
-                jj_pf[r][c] = (ecell && jin) ? Jb[c * (int)jstride + jq] : (R)0;
+                hold_pf[r][c] = ldraw<S>(hin[c], q);
+                mat_pf[r][c] = (int)A.idx[c][q];
+                if (ade1) jj_pf[r][c] = Jb[c * (int)jstride + jq];
             }
         }
     };
@@ -316,10 +288,10 @@ __device__ __forceinline__ void yee_tile(const YeeArgs<R>& A, int tx, int ty, in
         S e3[RPW][3], r3[3], c3 = zs;
         R r3w[3], clw = 0;
         load_plane(kstart, e3, r3, c3);
-        widen(e3, r3, c3, e_own, r3w, clw, kstart < Z);
+        widen(e3, r3, c3, e_own, r3w, clw, kstart);
         stage(e_own, r3w, clw);
         load_plane(kstart + 1, e3, r3, c3);
-        widen(e3, r3, c3, e_nx, row_nx, col_nx, kstart + 1 < Z);
+        widen(e3, r3, c3, e_nx, row_nx, col_nx, kstart + 1);
     }
     load_cells(kstart);
 
@@ -341,13 +313,69 @@ __device__ __forceinline__ void yee_tile(const YeeArgs<R>& A, int tx, int ty, in
                 mat[r][c] = mat_pf[r][c];
                 jcur[r][c] = jj_pf[r][c];
             }
-        if (!GEN || k + 2 <= kend) load_plane(k + 2, e_pf, row_pf, col_pf);  // H(kend-1) needs E(kend)
-        load_cells(k + 1);
         const R ipz = ldu(A.ip[2], k), idz = ldu(A.id[2], k);
         Slab<R> szE, szH;
         if constexpr (GEN || ZS) {
             szE = slab_at(A, 2, false, (!GEN || slab_z) ? k : -1);
             szH = slab_at(A, 2, true, (!GEN || slab_z) ? k : -1);
+        }
+        // CPML psi^n of plane k, loaded before the prefetch is issued (a load issued after it
+        // would make its first use wait for the whole prefetch): H side for every cell of the
+        // tile inside the storage, E side for the cells the tile updates. General tiles load
+        // the E side and issue the prefetch after the H update, so that the two psi sets are
+        // not held in registers at once.
+        R pH[RPW][6], pE[RPW][6];
+        auto load_psi_e = [&]() {
+#pragma unroll
+            for (int r = 0; r < RPW; ++r) {
+                const int lj = wv * RPW + r, j = jr[r];
+                if (!own_k || lane < 1 || lj < 1) continue;
+                if (GEN && (i > A.nx + 1 || j > A.ny + 1)) continue;
+                if (GEN && syE[r].s >= 0) {
+                    const long long q = psi_at(A, 1, syE[r].s, b, i, j, k);
+                    pE[r][0] = A.psiE[A.poff[0][0] + q];
+                    pE[r][1] = A.psiE[A.poff[2][1] + q];
+                }
+                if (szE.s >= 0) {
+                    const long long q = psi_at(A, 2, szE.s, b, i, j, k);
+                    pE[r][2] = A.psiE[A.poff[0][1] + q];
+                    pE[r][3] = A.psiE[A.poff[1][0] + q];
+                }
+                if (GEN && sxE.s >= 0) {
+                    const long long q = psi_at(A, 0, sxE.s, b, i, j, k);
+                    pE[r][4] = A.psiE[A.poff[1][1] + q];
+                    pE[r][5] = A.psiE[A.poff[2][0] + q];
+                }
+            }
+        };
+        if constexpr (GEN || ZS) {
+#pragma unroll
+            for (int r = 0; r < RPW; ++r) {
+                const int j = jr[r];
+#pragma unroll
+                for (int t = 0; t < 6; ++t) pH[r][t] = pE[r][t] = (R)0;
+                if (GEN && !(i < X && j < Y)) continue;
+                if (GEN && syH[r].s >= 0) {
+                    const long long q = psi_at(A, 1, syH[r].s, b, i, j, k);
+                    pH[r][0] = A.psiHin[A.poff[0][0] + q];
+                    pH[r][1] = A.psiHin[A.poff[2][1] + q];
+                }
+                if (szH.s >= 0) {
+                    const long long q = psi_at(A, 2, szH.s, b, i, j, k);
+                    pH[r][2] = A.psiHin[A.poff[0][1] + q];
+                    pH[r][3] = A.psiHin[A.poff[1][0] + q];
+                }
+                if (GEN && sxH.s >= 0) {
+                    const long long q = psi_at(A, 0, sxH.s, b, i, j, k);
+                    pH[r][4] = A.psiHin[A.poff[1][1] + q];
+                    pH[r][5] = A.psiHin[A.poff[2][0] + q];
+                }
+            }
+        }
+        if constexpr (ZS) load_psi_e();
+        if constexpr (!GEN) {
+            load_plane(k + 2, e_pf, row_pf, col_pf);  // H(kend-1) needs E(kend)
+            load_cells(k + 1);
         }
         __syncthreads();  // LDS E slot holds E^n(k); sH is free
         // ---- H^{n+1/2}(k):  H_c += dtmu (d1' - d0')  (pml.py H_TERMS signs)
@@ -370,29 +398,26 @@ __device__ __forceinline__ void yee_tile(const YeeArgs<R>& A, int tx, int ty, in
                 const bool vz = i >= 1 && i <= A.nx && j >= 1 && j <= A.ny;
                 // Hx: t0 along y, t1 along z; Hy: t0 along z, t1 along x; Hz: t0 along x, t1 along y
                 if (syH[r].s >= 0) {
-                    cpml_term(d[0][0], syH[r], A.psiHin + A.poff[0][0], A.psiHout + A.poff[0][0],
-                              psi_at(A, 1, syH[r].s, b, i, j, k), own && vx);
-                    cpml_term(d[2][1], syH[r], A.psiHin + A.poff[2][1], A.psiHout + A.poff[2][1],
-                              psi_at(A, 1, syH[r].s, b, i, j, k), own && vz);
+                    const long long q = psi_at(A, 1, syH[r].s, b, i, j, k);
+                    cpml_term(d[0][0], syH[r], pH[r][0], A.psiHout + A.poff[0][0], q, own && vx);
+                    cpml_term(d[2][1], syH[r], pH[r][1], A.psiHout + A.poff[2][1], q, own && vz);
                 }
                 if (szH.s >= 0) {
-                    cpml_term(d[0][1], szH, A.psiHin + A.poff[0][1], A.psiHout + A.poff[0][1],
-                              psi_at(A, 2, szH.s, b, i, j, k), own && vx);
-                    cpml_term(d[1][0], szH, A.psiHin + A.poff[1][0], A.psiHout + A.poff[1][0],
-                              psi_at(A, 2, szH.s, b, i, j, k), own && vy);
+                    const long long q = psi_at(A, 2, szH.s, b, i, j, k);
+                    cpml_term(d[0][1], szH, pH[r][2], A.psiHout + A.poff[0][1], q, own && vx);
+                    cpml_term(d[1][0], szH, pH[r][3], A.psiHout + A.poff[1][0], q, own && vy);
                 }
                 if (sxH.s >= 0) {
-                    cpml_term(d[1][1], sxH, A.psiHin + A.poff[1][1], A.psiHout + A.poff[1][1],
-                              psi_at(A, 0, sxH.s, b, i, j, k), own && vy);
-                    cpml_term(d[2][0], sxH, A.psiHin + A.poff[2][0], A.psiHout + A.poff[2][0],
-                              psi_at(A, 0, sxH.s, b, i, j, k), own && vz);
+                    const long long q = psi_at(A, 0, sxH.s, b, i, j, k);
+                    cpml_term(d[1][1], sxH, pH[r][4], A.psiHout + A.poff[1][1], q, own && vy);
+                    cpml_term(d[2][0], sxH, pH[r][5], A.psiHout + A.poff[2][0], q, own && vz);
                 }
             }
             if (ZS && szH.s >= 0) {  // Hx: t1 along z; Hy: t0 along z (x, y are interior)
                 const bool own = own_k && lane >= 1 && lj >= 1;
                 const long long pq = psi_at(A, 2, szH.s, b, i, j, k);
-                cpml_term(d[0][1], szH, A.psiHin + A.poff[0][1], A.psiHout + A.poff[0][1], pq, own);
-                cpml_term(d[1][0], szH, A.psiHin + A.poff[1][0], A.psiHout + A.poff[1][0], pq, own);
+                cpml_term(d[0][1], szH, pH[r][2], A.psiHout + A.poff[0][1], pq, own);
+                cpml_term(d[1][0], szH, pH[r][3], A.psiHout + A.poff[1][0], pq, own);
             }
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
@@ -500,6 +525,11 @@ __device__ __forceinline__ void yee_tile(const YeeArgs<R>& A, int tx, int ty, in
                 }
             }
         }
+        if constexpr (GEN) {
+            load_psi_e();
+            load_plane(k + 2, e_pf, row_pf, col_pf);  // H(kend-1) needs E(kend)
+            load_cells(k + 1);
+        }
         __syncthreads();  // H(k) complete in LDS; the E slot may be overwritten
         stage(e_nx, row_nx, col_nx);
         // ---- E^{n+1}(k) of own cells:  E_c = ca E_c + cb (d0' - d1')  (pml.py E_TERMS signs)
@@ -520,24 +550,24 @@ __device__ __forceinline__ void yee_tile(const YeeArgs<R>& A, int tx, int ty, in
                     // Ex: t0 along y, t1 along z; Ey: t0 along z, t1 along x; Ez: t0 along x, t1 along y
                     if (syE[r].s >= 0) {
                         const long long q = psi_at(A, 1, syE[r].s, b, i, j, k);
-                        cpml_term(d[0][0], syE[r], A.psiE + A.poff[0][0], A.psiE + A.poff[0][0], q, true);
-                        cpml_term(d[2][1], syE[r], A.psiE + A.poff[2][1], A.psiE + A.poff[2][1], q, true);
+                        cpml_term(d[0][0], syE[r], pE[r][0], A.psiE + A.poff[0][0], q, true);
+                        cpml_term(d[2][1], syE[r], pE[r][1], A.psiE + A.poff[2][1], q, true);
                     }
                     if (szE.s >= 0) {
                         const long long q = psi_at(A, 2, szE.s, b, i, j, k);
-                        cpml_term(d[0][1], szE, A.psiE + A.poff[0][1], A.psiE + A.poff[0][1], q, true);
-                        cpml_term(d[1][0], szE, A.psiE + A.poff[1][0], A.psiE + A.poff[1][0], q, true);
+                        cpml_term(d[0][1], szE, pE[r][2], A.psiE + A.poff[0][1], q, true);
+                        cpml_term(d[1][0], szE, pE[r][3], A.psiE + A.poff[1][0], q, true);
                     }
                     if (sxE.s >= 0) {
                         const long long q = psi_at(A, 0, sxE.s, b, i, j, k);
-                        cpml_term(d[1][1], sxE, A.psiE + A.poff[1][1], A.psiE + A.poff[1][1], q, true);
-                        cpml_term(d[2][0], sxE, A.psiE + A.poff[2][0], A.psiE + A.poff[2][0], q, true);
+                        cpml_term(d[1][1], sxE, pE[r][4], A.psiE + A.poff[1][1], q, true);
+                        cpml_term(d[2][0], sxE, pE[r][5], A.psiE + A.poff[2][0], q, true);
                     }
                 }
                 if (ZS && szE.s >= 0) {  // Ex: t1 along z; Ey: t0 along z
                     const long long pq = psi_at(A, 2, szE.s, b, i, j, k);
-                    cpml_term(d[0][1], szE, A.psiE + A.poff[0][1], A.psiE + A.poff[0][1], pq, true);
-                    cpml_term(d[1][0], szE, A.psiE + A.poff[1][0], A.psiE + A.poff[1][0], pq, true);
+                    cpml_term(d[0][1], szE, pE[r][2], A.psiE + A.poff[0][1], pq, true);
+                    cpml_term(d[1][0], szE, pE[r][3], A.psiE + A.poff[1][0], pq, true);
                 }
                 const int q = k * sxy + j * X + i;
 #pragma unroll
@@ -578,7 +608,7 @@ __device__ __forceinline__ void yee_tile(const YeeArgs<R>& A, int tx, int ty, in
                 hprev[r][c] = hcur[r][c];
                 e_own[r][c] = e_nx[r][c];
             }
-        widen(e_pf, row_pf, col_pf, e_nx, row_nx, col_nx, k + 2 < Z);
+        widen(e_pf, row_pf, col_pf, e_nx, row_nx, col_nx, k + 2);
     }
 }
 
