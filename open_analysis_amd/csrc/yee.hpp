@@ -8,18 +8,21 @@
 // A workgroup of NW wavefronts owns a (TXH-1) x (NW*RPW-1) tile of E cells and marches up one
 // z chunk. Per plane k it
 //   1. holds E^n(k) of the tile plus a one-cell high-side halo in LDS (one slot) and E^n(k),
-//      E^n(k+1) of its own cells in registers; E^n(k+2), and H^{n-1/2}, the material index and
-//      the Debye current of plane k+1, are already in flight;
+//      E^n(k+1) of its own cells in registers, and issues the loads of E^n(k+2) and of
+//      H^{n-1/2}, the material index and the Debye current of plane k+1 (general tiles after
+//      step 2), which are waited for only at the end of the plane;
 //   2. computes H^{n+1/2}(k) for the tile and its low-side halo row/column (recomputed, never
 //      exchanged) and stages it in LDS;
 //   3. overwrites the LDS slot with E^n(k+1) and advances E(k) of its own cells;
 // with two barriers per plane. H(k-1) stays in registers.
 //
-// Two instantiations per storage type: the interior kernel (GEN = false) runs on tiles that
-// touch no CPML slab and no wall and has no boundary code; the general kernel (GEN = true) adds
-// the CPML psi recursions (psi in slab arrays addressed by a 1-D slot map per axis, with the slot
-// and coefficient lookups hoisted out of the z loop), PMC mirror ghosts and the write masks of
-// the staggered lattice. Workgroups are mapped to tiles so that consecutive tiles (neighbours in
+// Three kinds of tile per storage type: the interior kernel (GEN = false, ZS = false) runs on
+// tiles that touch no CPML slab and no wall and has no boundary code; the z-slab kernel
+// (ZS = true) adds the z CPML only; the general kernel (GEN = true) adds the CPML psi recursions
+// of every axis (psi in slab arrays addressed by a 1-D slot map per axis, with the slot and
+// coefficient lookups hoisted out of the z loop), PMC mirror ghosts and the write masks of the
+// staggered lattice. Each reads the material coefficients from an LDS copy of the table (TAB)
+// or, for tables over NTAB rows, from memory. Workgroups are mapped to tiles so that consecutive tiles (neighbours in
 // x, which share the cache lines their rows straddle, or in y, which share E halo rows; the
 // autotuner picks the order) run on the same XCD at the same time. The library is built with
 // -ffp-contract=off: every kernel and tile shape computes a cell (and a recomputed halo cell)
